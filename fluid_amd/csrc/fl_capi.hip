@@ -1,0 +1,861 @@
+// libfluidhip host runtime: the C ABI of include/fluid_hip.h.
+//
+// A context owns the device copies of the reference's eight buffers (fluid.go:17-23)
+// and a host mirror of S. Mutating API calls (SetSolid, SetVelocity, AddSmoke,
+// ApplyForce...) are recorded as ordered word-level edits and applied on the device
+// by one kernel before the next device operation, so their effect and order match the
+// reference's direct slice writes exactly. Simulate enqueues the step kernels on the
+// context's stream in the reference's pass order (fluid.go:79-109).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "fl_internal.h"
+
+namespace fl {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static int err(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+static float go_minf_h(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  if (a < b) return a;
+  if (b < a) return b;
+  return std::signbit(a) ? a : b;
+}
+static float go_maxf_h(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  if (a > b) return a;
+  if (b > a) return b;
+  return std::signbit(a) ? b : a;
+}
+
+static int check_xy(Ctx* c, int i, int j) {
+  if (i < 0 || i >= c->NX) return err(FL_ERR_X_RANGE, "invalid x-index: " + std::to_string(i));
+  if (j < 0 || j >= c->NY) return err(FL_ERR_Y_RANGE, "invalid y-index: " + std::to_string(j));
+  return FL_OK;
+}
+
+static void push_op(Ctx* c, int field, size_t idx, bool add, float v) {
+  WordOp op;
+  op.key = (uint32_t)(field * c->N + idx) | (add ? 0x80000000u : 0u);
+  op.kind = add ? 1 : 0;
+  op.val = v;
+  c->edits.push_back(op);
+}
+
+// Apply queued edits on the device: ops are grouped per target word, order kept.
+static int flush_edits(Ctx* c) {
+  if (c->edits.empty()) return FL_OK;
+  const size_t n = c->edits.size();
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    return (c->edits[a].key & 0x7fffffffu) < (c->edits[b].key & 0x7fffffffu);
+  });
+  std::vector<uint32_t> keys(n);
+  std::vector<float> vals(n);
+  std::vector<int32_t> seg;
+  seg.reserve(n + 1);
+  uint32_t prev = 0xffffffffu;
+  for (size_t k = 0; k < n; k++) {
+    const WordOp& op = c->edits[order[k]];
+    keys[k] = op.key;
+    vals[k] = op.val;
+    if ((op.key & 0x7fffffffu) != prev) {
+      seg.push_back((int32_t)k);
+      prev = op.key & 0x7fffffffu;
+    }
+  }
+  const int nseg = (int)seg.size();
+  seg.push_back((int32_t)n);
+  if (n > c->edit_cap) {
+    if (c->d_edit_keys) FL_HIP(hipFree(c->d_edit_keys));
+    if (c->d_edit_vals) FL_HIP(hipFree(c->d_edit_vals));
+    if (c->d_edit_seg) FL_HIP(hipFree(c->d_edit_seg));
+    size_t cap = std::max<size_t>(n, 1024);
+    FL_HIP(hipMalloc(&c->d_edit_keys, 4 * cap));
+    FL_HIP(hipMalloc(&c->d_edit_vals, 4 * cap));
+    FL_HIP(hipMalloc(&c->d_edit_seg, 4 * (cap + 1)));
+    c->edit_cap = cap;
+  }
+  FL_HIP(hipMemcpyAsync(c->d_edit_keys, keys.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipMemcpyAsync(c->d_edit_vals, vals.data(), 4 * n, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipMemcpyAsync(c->d_edit_seg, seg.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, c->stream));
+  int rc = launch_apply_edits(c, nseg);
+  if (rc) return rc;
+  FL_HIP(hipStreamSynchronize(c->stream));  // host staging vectors die here
+  c->edits.clear();
+  return FL_OK;
+}
+
+static int prepare(Ctx* c) {
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  if (c->flags_dirty) return launch_flags(c);
+  return FL_OK;
+}
+
+// Per-step source program: sources, sinks, then the jet (main.go:231-241,474-486).
+static int build_sources(Ctx* c) {
+  if (!c->src_dirty) return FL_OK;
+  std::vector<SrcOp> ops;
+  auto inb = [&](int i, int j) { return i >= 0 && i < c->NX && j >= 0 && j < c->NY; };
+  for (auto& s : c->sources) {
+    if (!inb(s.I, s.J)) continue;
+    int cell = s.I * c->NY + s.J;
+    ops.push_back(SrcOp{cell, 0, 1, s.U, s.V});
+    ops.push_back(SrcOp{cell, 1, 1, 0.5f, 0.0f});
+  }
+  for (auto& s : c->sinks) {
+    if (!inb(s.I, s.J)) continue;
+    ops.push_back(SrcOp{s.I * c->NY + s.J, 0, 1, s.U, s.V});
+  }
+  if (c->jet) {
+    for (int j = c->jetLo; j < c->jetHi; j++) {
+      int cell = c->jetX * c->NY + j;
+      ops.push_back(SrcOp{cell, 0, 0, c->jetU, 0.0f});
+      ops.push_back(SrcOp{cell, 1, 0, c->jetSmoke, 0.0f});
+    }
+  }
+  std::stable_sort(ops.begin(), ops.end(), [](const SrcOp& a, const SrcOp& b) { return a.cell < b.cell; });
+  std::vector<int32_t> seg;
+  for (size_t k = 0; k < ops.size(); k++)
+    if (k == 0 || ops[k].cell != ops[k - 1].cell) seg.push_back((int32_t)k);
+  const int nseg = (int)seg.size();
+  seg.push_back((int32_t)ops.size());
+  if (ops.size() > c->src_cap) {
+    if (c->d_src) FL_HIP(hipFree(c->d_src));
+    if (c->d_src_seg) FL_HIP(hipFree(c->d_src_seg));
+    size_t cap = std::max<size_t>(ops.size(), 256);
+    FL_HIP(hipMalloc(&c->d_src, sizeof(SrcOp) * cap));
+    FL_HIP(hipMalloc(&c->d_src_seg, 4 * (cap + 1)));
+    c->src_cap = cap;
+  }
+  if (!ops.empty()) {
+    FL_HIP(hipMemcpyAsync(c->d_src, ops.data(), sizeof(SrcOp) * ops.size(), hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipMemcpyAsync(c->d_src_seg, seg.data(), 4 * (nseg + 1), hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipStreamSynchronize(c->stream));
+  }
+  c->n_src_ops = (int)ops.size();
+  c->n_src_seg = nseg;
+  c->src_dirty = false;
+  return FL_OK;
+}
+
+// ---- the step ---------------------------------------------------------------------------
+static int make_incompressible(Ctx* c, int iters, float dt, bool copy_border) {
+  int rc;
+  if (copy_border) {
+    // copyBorder(newU,U), copyBorder(newV,V) (fluid.go:145-146). Inside Simulate the
+    // velocity advection rewrites newU/newV's ring before anything reads it, so the
+    // step skips these two copies.
+    if ((rc = launch_copy_border(c, c->nU, c->U))) return rc;
+    if ((rc = launch_copy_border(c, c->nV, c->V))) return rc;
+  }
+  if (c->prm.use_multigrid && c->prm.multigrid_levels > 1)
+    return err(FL_ERR_UNSUPPORTED, "UseMultigrid: the multigrid V-cycle is not implemented on the device");
+  return pressure_solve(c, iters, dt);
+}
+
+static int advect_velocity_sl(Ctx* c, float dt) {
+  int rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt);
+  if (rc) return rc;
+  std::swap(c->U, c->U2);
+  std::swap(c->V, c->V2);
+  return FL_OK;
+}
+static int advect_smoke_sl(Ctx* c, float dt) {
+  int rc = launch_advect_smoke(c, c->M, c->M2, dt);
+  if (rc) return rc;
+  std::swap(c->M, c->M2);
+  return FL_OK;
+}
+// advectVelocityBFECC (fluid.go:911-994): forward into U2/V2 (also newU/newV), backward +
+// correction + clamp into T0/T1, final forward pass from the corrected field into U/V.
+static int advect_velocity_bfecc(Ctx* c, float dt) {
+  int rc;
+  if ((rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt))) return rc;
+  if ((rc = launch_bfecc_velocity_corr(c, c->U, c->V, c->U2, c->V2, c->T0, c->T1, dt))) return rc;
+  return launch_advect_velocity(c, c->T0, c->T1, c->U, c->V, dt);
+}
+// advectSmokeBFECC (fluid.go:997-1051)
+static int advect_smoke_bfecc(Ctx* c, float dt) {
+  int rc;
+  if ((rc = launch_advect_smoke(c, c->M, c->M2, dt))) return rc;
+  if ((rc = launch_bfecc_smoke_corr(c, c->M, c->M2, c->T2, dt))) return rc;
+  return launch_advect_smoke(c, c->T2, c->M, dt);
+}
+
+static int simulate(Ctx* c, float dt) {
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (c->NX < 3 || c->NY < 3) return err(FL_ERR_UNSUPPORTED, "grid must have at least 1x1 interior cells");
+  const fl_params& P = c->prm;
+  if ((rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;                      // fill(p,0)
+  if (P.viscosity_diffusion > 0 && (rc = launch_viscosity(c, dt))) return rc;          // applyViscosity
+  if ((rc = make_incompressible(c, P.pressure_iters, dt, false))) return rc;           // makeIncompressible
+  const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
+  if (conf && (rc = launch_curl(c, c->T0))) return rc;
+  if ((conf || turb) && (rc = launch_conf_turb(c, c->T0, dt, conf, turb))) return rc;  // confinement, turbulence
+  if ((rc = launch_borders(c))) return rc;                                             // handleBorders
+  if (P.use_bfecc) {
+    if ((rc = advect_velocity_bfecc(c, dt))) return rc;
+    return advect_smoke_bfecc(c, dt);
+  }
+  if ((rc = advect_velocity_sl(c, dt))) return rc;
+  return advect_smoke_sl(c, dt);
+}
+
+static void free_ctx(Ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->dev);
+  float** bufs[] = {&c->U, &c->V, &c->nU, &c->nV, &c->p, &c->S, &c->M, &c->nM, &c->U2, &c->V2,
+                    &c->M2, &c->T0, &c->T1, &c->T2, &c->T3, &c->d_omega, &c->d_edit_vals, &c->d_xy};
+  for (auto** b : bufs)
+    if (*b) (void)hipFree(*b);
+  if (c->flags) (void)hipFree(c->flags);
+  if (c->flags_diag) (void)hipFree(c->flags_diag);
+  if (c->d_ctrl) (void)hipFree(c->d_ctrl);
+  if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
+  if (c->d_red) (void)hipFree(c->d_red);
+  if (c->d_edit_keys) (void)hipFree(c->d_edit_keys);
+  if (c->d_edit_seg) (void)hipFree(c->d_edit_seg);
+  if (c->d_src) (void)hipFree(c->d_src);
+  if (c->d_src_seg) (void)hipFree(c->d_src_seg);
+  pressure_free(c);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static fl_params default_params() {
+  fl_params p{};
+  p.confinement = 0.0f;
+  p.viscosity_diffusion = 0.0f;
+  p.pressure_damping = 1.0f;
+  p.turbulence_strength = 0.02f;
+  p.smoke_advection = 1.0f;
+  p.relaxation = 1.9f;
+  p.use_multigrid = 0;
+  p.multigrid_levels = 2;
+  p.use_bfecc = 0;
+  p.pressure_iters = 8;
+  p.early_exit = 1;
+  p.solver = 0;
+  p.block_sweeps = 0;
+  return p;
+}
+
+static int set_solid(Ctx* c, int i, int j, bool solid) {
+  int rc = check_xy(c, i, j);
+  if (rc) return rc;
+  const size_t cell = (size_t)i * c->NY + j;
+  const float sv = solid ? 0.0f : 1.0f;
+  c->hS[cell] = sv;
+  push_op(c, FL_FIELD_S, cell, false, sv);
+  c->flags_dirty = true;
+  if (solid) {
+    const int n = c->NY;
+    for (int f : {FL_FIELD_U, FL_FIELD_NEWU}) {
+      push_op(c, f, cell, false, 0.0f);
+      if (i + 1 < c->NX) push_op(c, f, cell + n, false, 0.0f);
+    }
+    for (int f : {FL_FIELD_V, FL_FIELD_NEWV}) {
+      push_op(c, f, cell, false, 0.0f);
+      if (j + 1 < c->NY) push_op(c, f, cell + 1, false, 0.0f);
+    }
+  }
+  return FL_OK;
+}
+
+static void apply_force(Ctx* c, int i, int j, float fx, float fy) {
+  if (i < 1 || i >= c->NX - 1 || j < 1 || j >= c->NY - 1) return;
+  const size_t cell = (size_t)i * c->NY + j;
+  if (c->hS[cell] == 0.0f) return;
+  push_op(c, FL_FIELD_U, cell, true, fx);
+  push_op(c, FL_FIELD_V, cell, true, fy);
+}
+
+static void apply_wall_settings(Ctx* c) {
+  for (int i = 0; i < c->NX; i++) {
+    set_solid(c, i, 0, c->wallBottom);
+    set_solid(c, i, c->NY - 1, c->wallTop);
+  }
+  for (int j = 0; j < c->NY; j++) {
+    set_solid(c, 0, j, c->wallLeft);
+    set_solid(c, c->NX - 1, j, c->wallRight);
+  }
+}
+
+static int reset(Ctx* c) {
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  for (float* b : {c->U, c->V, c->nU, c->nV, c->p, c->M, c->nM})
+    if ((rc = launch_fill(b, 0.0f, c->N, c->stream))) return rc;
+  return FL_OK;
+}
+
+static void set_circular_obstacle(Ctx* c, int cx, int cy, int r) {
+  for (int i = cx - r; i <= cx + r; i++)
+    for (int j = cy - r; j <= cy + r; j++) {
+      if (i < 0 || i >= c->NX || j < 0 || j >= c->NY) continue;
+      float dx = (float)(i - cx), dy = (float)(j - cy);
+      if (dx * dx + dy * dy <= (float)(r * r)) set_solid(c, i, j, true);
+    }
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+#define CTX_CHECK(c)                                                                                   \
+  do {                                                                                                 \
+    if (!(c)) return err(FL_ERR_INVALID_ARG, "null context");                                          \
+    hipError_t e__ = hipSetDevice(reinterpret_cast<Ctx*>(c)->dev);                                      \
+    if (e__ != hipSuccess) return err(FL_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e__)); \
+  } while (0)
+#define C_OF(x) reinterpret_cast<Ctx*>(x)
+
+extern "C" {
+
+const char* fl_last_error(void) { return g_err.c_str(); }
+
+int fl_device_count(int* n) {
+  int k = 0;
+  hipError_t e = hipGetDeviceCount(&k);
+  if (e != hipSuccess) k = 0;
+  if (n) *n = k;
+  return FL_OK;
+}
+
+int fl_create(float density, int width, int height, float h, fl_ctx** out) {
+  if (!out) return err(FL_ERR_INVALID_ARG, "null output pointer");
+  *out = nullptr;
+  if (width < 1 || height < 1) return err(FL_ERR_INVALID_ARG, "width and height must be >= 1");
+  const int64_t cells = (int64_t)(width + 2) * (height + 2);
+  if (cells > (int64_t)1 << 28) return err(FL_ERR_INVALID_ARG, "grid too large (more than 2^28 cells)");
+  if (!(h > 0.0f)) return err(FL_ERR_INVALID_ARG, "h must be > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return err(FL_ERR_NO_DEVICE, "no HIP device available: libfluidhip requires an MI355X (gfx950) GPU");
+  Ctx* c = new Ctx();
+  if (hipGetDevice(&c->dev) != hipSuccess) c->dev = 0;
+  c->NX = width + 2;
+  c->NY = height + 2;
+  c->N = (size_t)c->NX * c->NY;
+  c->density = density;
+  c->h = h;
+  c->prm = default_params();
+  c->grid.NX = c->NX;
+  c->grid.NY = c->NY;
+  c->grid.h = h;
+  c->grid.h1 = 1.0f / h;
+  c->grid.hs = 0.5f * h;
+  c->grid.ha = h / 2.0f;
+  c->grid.xmax = (float)c->NX * h;
+  c->grid.ymax = (float)c->NY * h;
+  c->hS.assign(c->N, 0.0f);
+  auto fail = [&](hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    free_ctx(c);
+    return (int)FL_ERR_HIP;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(e, "hipStreamCreate");
+  float** bufs[] = {&c->U, &c->V, &c->nU, &c->nV, &c->p, &c->S, &c->M, &c->nM,
+                    &c->U2, &c->V2, &c->M2, &c->T0, &c->T1, &c->T2, &c->T3};
+  for (auto** b : bufs) {
+    if ((e = hipMalloc(b, sizeof(float) * c->N)) != hipSuccess) return fail(e, "hipMalloc field");
+    if ((e = hipMemsetAsync(*b, 0, sizeof(float) * c->N, c->stream)) != hipSuccess) return fail(e, "hipMemset");
+  }
+  if ((e = hipMalloc(&c->flags, c->N)) != hipSuccess) return fail(e, "hipMalloc flags");
+  if ((e = hipMalloc(&c->flags_diag, (size_t)(c->NX + c->NY - 1) * c->NX)) != hipSuccess)
+    return fail(e, "hipMalloc flags_diag");
+  if ((e = hipMalloc(&c->d_ctrl, 64)) != hipSuccess) return fail(e, "hipMalloc ctrl");
+  if ((e = hipMemsetAsync(c->d_ctrl, 0, 64, c->stream)) != hipSuccess) return fail(e, "hipMemset ctrl");
+  if ((e = hipMalloc(&c->d_red, 64)) != hipSuccess) return fail(e, "hipMalloc red");
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+  c->flags_dirty = true;
+  *out = reinterpret_cast<fl_ctx*>(c);
+  return FL_OK;
+}
+
+int fl_destroy(fl_ctx* ctx) {
+  if (!ctx) return FL_OK;
+  free_ctx(C_OF(ctx));
+  return FL_OK;
+}
+
+int fl_dims(fl_ctx* ctx, int* nx, int* ny) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  if (nx) *nx = C_OF(ctx)->NX;
+  if (ny) *ny = C_OF(ctx)->NY;
+  return FL_OK;
+}
+int fl_get_h(fl_ctx* ctx, float* h) {
+  if (!ctx || !h) return err(FL_ERR_INVALID_ARG, "null argument");
+  *h = C_OF(ctx)->h;
+  return FL_OK;
+}
+int fl_get_params(fl_ctx* ctx, fl_params* out) {
+  if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
+  *out = C_OF(ctx)->prm;
+  return FL_OK;
+}
+int fl_set_params(fl_ctx* ctx, const fl_params* in) {
+  if (!ctx || !in) return err(FL_ERR_INVALID_ARG, "null argument");
+  if (in->pressure_iters < 0 || in->pressure_iters > 100000)
+    return err(FL_ERR_INVALID_ARG, "pressure_iters out of range");
+  if (in->solver < 0 || in->solver > 2) return err(FL_ERR_INVALID_ARG, "solver must be 0, 1 or 2");
+  C_OF(ctx)->prm = *in;
+  return FL_OK;
+}
+
+static float* field_ptr(Ctx* c, int f) {
+  switch (f) {
+    case FL_FIELD_U: return c->U;
+    case FL_FIELD_V: return c->V;
+    case FL_FIELD_NEWU: return c->nU;
+    case FL_FIELD_NEWV: return c->nV;
+    case FL_FIELD_P: return c->p;
+    case FL_FIELD_S: return c->S;
+    case FL_FIELD_M: return c->M;
+    case FL_FIELD_NEWM: return c->nM;
+  }
+  return nullptr;
+}
+
+int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  float* dst = field_ptr(c, field);
+  if (!dst || !src) return err(FL_ERR_INVALID_ARG, "bad field or null source");
+  if (n != c->N) return err(FL_ERR_INVALID_ARG, "size must be NumX*NumY");
+  if (field == FL_FIELD_S) {
+    for (size_t k = 0; k < n; k++) {
+      uint32_t b;
+      std::memcpy(&b, &src[k], 4);
+      if (b != 0u && b != 0x3f800000u)
+        return err(FL_ERR_INVALID_ARG, "S values must be exactly 0.0 (solid) or 1.0 (fluid)");
+    }
+  }
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  FL_HIP(hipMemcpyAsync(dst, src, 4 * n, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  if (field == FL_FIELD_S) {
+    std::memcpy(c->hS.data(), src, 4 * n);
+    c->flags_dirty = true;
+  }
+  return FL_OK;
+}
+
+int fl_download(fl_ctx* ctx, int field, float* dst, size_t n) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  float* src = field_ptr(c, field);
+  if (!src || !dst) return err(FL_ERR_INVALID_ARG, "bad field or null destination");
+  if (n != c->N) return err(FL_ERR_INVALID_ARG, "size must be NumX*NumY");
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  FL_HIP(hipMemcpyAsync(dst, src, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  return FL_OK;
+}
+
+int fl_simulate(fl_ctx* ctx, float dt) {
+  CTX_CHECK(ctx);
+  return simulate(C_OF(ctx), dt);
+}
+
+int fl_sync(fl_ctx* ctx) {
+  CTX_CHECK(ctx);
+  FL_HIP(hipStreamSynchronize(C_OF(ctx)->stream));
+  return FL_OK;
+}
+
+int fl_get_stats(fl_ctx* ctx, fl_stats* out) {
+  CTX_CHECK(ctx);
+  if (!out) return err(FL_ERR_INVALID_ARG, "null output");
+  Ctx* c = C_OF(ctx);
+  FL_HIP(hipStreamSynchronize(c->stream));
+  int32_t ctrl[4] = {0, 0, 0, 0};
+  FL_HIP(hipMemcpy(ctrl, c->d_ctrl, 16, hipMemcpyDeviceToHost));
+  *out = fl_stats{};
+  out->invalid_sample = ctrl[2] != 0;
+  int n = c->last_n, kept = n;
+  if (c->host_sweeps >= 0) {
+    kept = c->host_sweeps;
+  } else if (ctrl[1] > 0) {
+    kept = ctrl[1];
+  }
+  out->sweeps_run = kept;
+  out->reran = kept < n;
+  if (kept > 0 && c->d_maxdiv) {
+    uint32_t bits = 0;
+    FL_HIP(hipMemcpy(&bits, c->d_maxdiv + (out->reran ? c->nmax : 0) + (kept - 1), 4, hipMemcpyDeviceToHost));
+    std::memcpy(&out->max_div_last, &bits, 4);
+  }
+  if (ctrl[3] != 0) return err(FL_ERR_INTERNAL, "pressure kernel reported a dependency wait timeout");
+  return FL_OK;
+}
+
+int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  FL_HIP(hipStreamSynchronize(c->stream));
+  int n = c->last_n;
+  if (count) *count = n;
+  if (out && n > 0 && c->d_maxdiv) {
+    int m = std::min(cap, n);
+    FL_HIP(hipMemcpy(out, c->d_maxdiv, 4 * m, hipMemcpyDeviceToHost));
+  }
+  return FL_OK;
+}
+
+int fl_set_solid(fl_ctx* ctx, int i, int j, int solid) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  return set_solid(C_OF(ctx), i, j, solid != 0);
+}
+int fl_is_solid(fl_ctx* ctx, int i, int j, int* out) {
+  if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
+  Ctx* c = C_OF(ctx);
+  int rc = check_xy(c, i, j);
+  if (rc) return rc;
+  *out = c->hS[(size_t)i * c->NY + j] == 0.0f;
+  return FL_OK;
+}
+int fl_set_velocity(fl_ctx* ctx, int i, int j, float u, float v) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  int rc = check_xy(c, i, j);
+  if (rc) return rc;
+  const size_t cell = (size_t)i * c->NY + j;
+  push_op(c, FL_FIELD_U, cell, false, u);
+  push_op(c, FL_FIELD_V, cell, false, v);
+  return FL_OK;
+}
+int fl_add_smoke(fl_ctx* ctx, int i, int j, float s) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  int rc = check_xy(c, i, j);
+  if (rc) return rc;
+  push_op(c, FL_FIELD_M, (size_t)i * c->NY + j, true, s);
+  return FL_OK;
+}
+int fl_reset(fl_ctx* ctx) {
+  CTX_CHECK(ctx);
+  return reset(C_OF(ctx));
+}
+int fl_apply_force(fl_ctx* ctx, int i, int j, float fx, float fy) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  apply_force(C_OF(ctx), i, j, fx, fy);
+  return FL_OK;
+}
+int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int radius) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  if (radius <= 0) {
+    apply_force(c, cx, cy, fx, fy);
+    return FL_OK;
+  }
+  const float r2 = (float)(radius * radius);
+  for (int i = cx - radius; i <= cx + radius; i++)
+    for (int j = cy - radius; j <= cy + radius; j++) {
+      if (i < 1 || i >= c->NX - 1 || j < 1 || j >= c->NY - 1) continue;
+      float dx = (float)(i - cx), dy = (float)(j - cy);
+      float d2 = dx * dx + dy * dy;
+      if (d2 > r2) continue;
+      float w = (float)std::exp((double)(-3.0f * d2 / r2));
+      apply_force(c, i, j, fx * w, fy * w);
+    }
+  return FL_OK;
+}
+int fl_set_circular_obstacle(fl_ctx* ctx, int cx, int cy, int radius) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  set_circular_obstacle(C_OF(ctx), cx, cy, radius);
+  return FL_OK;
+}
+
+int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out) {
+  CTX_CHECK(ctx);
+  if (!out) return err(FL_ERR_INVALID_ARG, "null output");
+  Ctx* c = C_OF(ctx);
+  int rc = prepare(c);
+  if (rc) return rc;
+  float maxVel = 0.0f;
+  if ((rc = launch_max_reduce(c, 1, &maxVel))) return rc;
+  if (maxVel == 0.0f) {
+    *out = basedt;
+    return FL_OK;
+  }
+  float adt = 0.8f * c->h / maxVel;
+  adt = go_maxf_h(go_minf_h(adt, basedt * 2.0f), basedt * 0.1f);
+  *out = adt;
+  return FL_OK;
+}
+int fl_max_divergence(fl_ctx* ctx, float* out) {
+  CTX_CHECK(ctx);
+  if (!out) return err(FL_ERR_INVALID_ARG, "null output");
+  Ctx* c = C_OF(ctx);
+  int rc = prepare(c);
+  if (rc) return rc;
+  return launch_max_reduce(c, 0, out);
+}
+
+int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int n, float* us, float* vs) {
+  CTX_CHECK(ctx);
+  if (n < 0 || (n > 0 && (!xs || !ys || !us || !vs))) return err(FL_ERR_INVALID_ARG, "bad arguments");
+  if (n == 0) return FL_OK;
+  Ctx* c = C_OF(ctx);
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (n > c->xy_cap) {
+    if (c->d_xy) FL_HIP(hipFree(c->d_xy));
+    FL_HIP(hipMalloc(&c->d_xy, sizeof(float) * 4 * n));
+    c->xy_cap = n;
+  }
+  std::vector<float> xy(2 * (size_t)n), uv(2 * (size_t)n);
+  for (int k = 0; k < n; k++) {
+    xy[2 * k] = xs[k];
+    xy[2 * k + 1] = ys[k];
+  }
+  FL_HIP(hipMemcpyAsync(c->d_xy, xy.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  if ((rc = launch_sample_batch(c, c->d_xy, n, c->d_xy + 2 * (size_t)n))) return rc;
+  FL_HIP(hipMemcpyAsync(uv.data(), c->d_xy + 2 * (size_t)n, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < n; k++) {
+    us[k] = uv[2 * k];
+    vs[k] = uv[2 * k + 1];
+  }
+  return FL_OK;
+}
+int fl_sample_velocity(fl_ctx* ctx, float x, float y, float* u, float* v) {
+  return fl_sample_velocity_batch(ctx, &x, &y, 1, u, v);
+}
+
+// Vorticity()/VelocityMagnitude(): field on the device, MinValue/MaxValue scanned in
+// the reference's loop order over interior fluid cells (fluid.go:810-828, 844-863).
+static int view(fl_ctx* ctx, int kind, float* out, size_t n, float* mn, float* mx) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  if (!out || n != c->N) return err(FL_ERR_INVALID_ARG, "output must hold NumX*NumY floats");
+  int rc = prepare(c);
+  if (rc) return rc;
+  if ((rc = launch_view(c, kind, c->T3))) return rc;
+  FL_HIP(hipMemcpyAsync(out, c->T3, 4 * n, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  float minV = std::numeric_limits<float>::max(), maxV = -std::numeric_limits<float>::max();
+  for (int i = 1; i < c->NX - 1; i++)
+    for (int j = 1; j < c->NY - 1; j++) {
+      size_t k = (size_t)i * c->NY + j;
+      if (c->hS[k] == 0.0f) continue;
+      if (out[k] < minV) minV = out[k];
+      if (out[k] > maxV) maxV = out[k];
+    }
+  if (mn) *mn = minV;
+  if (mx) *mx = maxV;
+  return FL_OK;
+}
+int fl_vorticity(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) { return view(ctx, 0, out, n, mn, mx); }
+int fl_velocity_magnitude(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) {
+  return view(ctx, 1, out, n, mn, mx);
+}
+// Pressure()/Smoke() MinValue/MaxValue over every cell in index order (pressure.go:6-15).
+int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  std::vector<float> h(c->N);
+  int rc = fl_download(ctx, field, h.data(), c->N);
+  if (rc) return rc;
+  float minV = std::numeric_limits<float>::max(), maxV = -std::numeric_limits<float>::max();
+  for (float v : h) {
+    if (v < minV) minV = v;
+    if (v > maxV) maxV = v;
+  }
+  if (mn) *mn = minV;
+  if (mx) *mx = maxV;
+  return FL_OK;
+}
+
+int fl_pass(fl_ctx* ctx, int pass, float dt, int iters) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  int rc = prepare(c);
+  if (rc) return rc;
+  switch (pass) {
+    case FL_PASS_COPY_BORDER_UV:
+      if ((rc = launch_copy_border(c, c->nU, c->U))) return rc;
+      return launch_copy_border(c, c->nV, c->V);
+    case FL_PASS_VISCOSITY:
+      if (c->prm.viscosity_diffusion <= 0) return FL_OK;
+      return launch_viscosity(c, dt);
+    case FL_PASS_MAKE_INCOMPRESSIBLE: return make_incompressible(c, iters, dt, true);
+    case FL_PASS_CONFINEMENT:
+      if ((rc = launch_curl(c, c->T0))) return rc;
+      return launch_conf_turb(c, c->T0, dt, 1, 0);
+    case FL_PASS_TURBULENCE:
+      if (c->prm.turbulence_strength <= 0) return FL_OK;
+      return launch_conf_turb(c, c->T0, dt, 0, 1);
+    case FL_PASS_BORDERS: return launch_borders(c);
+    case FL_PASS_ADVECT_VELOCITY: return advect_velocity_sl(c, dt);
+    case FL_PASS_ADVECT_SMOKE: return advect_smoke_sl(c, dt);
+    case FL_PASS_ADVECT_VELOCITY_BFECC: return advect_velocity_bfecc(c, dt);
+    case FL_PASS_ADVECT_SMOKE_BFECC: return advect_smoke_bfecc(c, dt);
+    case FL_PASS_CLEAR_P: return launch_fill(c->p, 0.0f, c->N, c->stream);
+  }
+  return err(FL_ERR_INVALID_ARG, "unknown pass");
+}
+
+// ---- scene layer (package main) ------------------------------------------------------------
+int fl_scene_set_walls(fl_ctx* ctx, int top, int bottom, int left, int right) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  c->wallTop = top;
+  c->wallBottom = bottom;
+  c->wallLeft = left;
+  c->wallRight = right;
+  return FL_OK;
+}
+int fl_scene_apply_walls(fl_ctx* ctx) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  apply_wall_settings(C_OF(ctx));
+  return FL_OK;
+}
+int fl_scene_clear_walls(fl_ctx* ctx) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++) set_solid(c, i, j, false);
+  apply_wall_settings(c);
+  return FL_OK;
+}
+int fl_scene_preset(fl_ctx* ctx, int preset, float lid_u) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  if (preset < 0 || preset > 2) return err(FL_ERR_INVALID_ARG, "preset must be 0 (jet), 1 (cavity) or 2 (karman)");
+  int rc = reset(c);
+  if (rc) return rc;
+  c->sources.clear();
+  c->sinks.clear();
+  c->src_dirty = true;
+  for (int i = 0; i < c->NX; i++)
+    for (int j = 0; j < c->NY; j++) set_solid(c, i, j, false);
+  if (preset == 0) {
+    c->wallTop = c->wallBottom = c->wallLeft = true;
+    c->wallRight = false;
+  } else if (preset == 1) {
+    c->wallTop = c->wallBottom = c->wallLeft = c->wallRight = true;
+    c->jet = false;
+    for (int i = 2; i < c->NX - 2; i++) c->sources.push_back(Source{i, c->NY - 2, lid_u, 0.0f});
+  } else {
+    c->wallTop = c->wallBottom = c->wallLeft = true;
+    c->wallRight = false;
+    set_circular_obstacle(c, c->NX / 4, c->NY / 2, 8);
+  }
+  apply_wall_settings(c);
+  return FL_OK;
+}
+int fl_scene_set_jet(fl_ctx* ctx, int on, int x, int j_lo, int j_hi, float u, float smoke) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  if (on) {
+    int rc = check_xy(c, x, 0);
+    if (rc) return rc;
+    if (j_lo < j_hi && ((rc = check_xy(c, x, j_lo)) || (rc = check_xy(c, x, j_hi - 1)))) return rc;
+  }
+  c->jet = on != 0;
+  c->jetX = x;
+  c->jetLo = j_lo;
+  c->jetHi = j_hi;
+  c->jetU = u;
+  c->jetSmoke = smoke;
+  c->src_dirty = true;
+  return FL_OK;
+}
+int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  (sink ? c->sinks : c->sources).push_back(Source{i, j, u, v});
+  c->src_dirty = true;
+  return FL_OK;
+}
+int fl_scene_clear_sources(fl_ctx* ctx) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  c->sources.clear();
+  c->sinks.clear();
+  c->src_dirty = true;
+  return FL_OK;
+}
+int fl_scene_apply_sources(fl_ctx* ctx) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  auto inb = [&](int i, int j) { return i >= 0 && i < c->NX && j >= 0 && j < c->NY; };
+  for (auto& s : c->sources)
+    if (inb(s.I, s.J) && c->hS[(size_t)s.I * c->NY + s.J] != 0.0f) {
+      fl_set_velocity(ctx, s.I, s.J, s.U, s.V);
+      fl_add_smoke(ctx, s.I, s.J, 0.5f);
+    }
+  for (auto& s : c->sinks)
+    if (inb(s.I, s.J) && c->hS[(size_t)s.I * c->NY + s.J] != 0.0f) fl_set_velocity(ctx, s.I, s.J, s.U, s.V);
+  return FL_OK;
+}
+int fl_scene_update(fl_ctx* ctx, float dt) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  if ((rc = build_sources(c))) return rc;
+  if ((rc = launch_apply_sources(c))) return rc;
+  return simulate(c, dt);
+}
+int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  int dx = std::abs(x1 - x0), dy = -std::abs(y1 - y0);
+  int sx = x0 < x1 ? 1 : -1, sy = y0 < y1 ? 1 : -1;
+  int e = dx + dy;
+  for (;;) {
+    if (x0 >= 0 && x0 < c->NX && y0 >= 0 && y0 < c->NY) set_solid(c, x0, y0, solid != 0);
+    if (x0 == x1 && y0 == y1) break;
+    int e2 = 2 * e;
+    if (e2 >= dy) {
+      e += dy;
+      x0 += sx;
+    }
+    if (e2 <= dx) {
+      e += dx;
+      y0 += sy;
+    }
+  }
+  return FL_OK;
+}
+int fl_scene_smoke_brush(fl_ctx* ctx, int ci, int cj, int r, float strength) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = C_OF(ctx);
+  for (int i = ci - r; i <= ci + r; i++)
+    for (int j = cj - r; j <= cj + r; j++) {
+      if (i < 0 || i >= c->NX || j < 0 || j >= c->NY) continue;
+      float dx = (float)(i - ci), dy = (float)(j - cj);
+      if (dx * dx + dy * dy > (float)(r * r)) continue;
+      fl_add_smoke(ctx, i, j, strength * 0.3f);
+    }
+  return FL_OK;
+}
+
+}  // extern "C"

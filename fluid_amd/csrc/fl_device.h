@@ -1,0 +1,137 @@
+// Device-side helpers shared by the step kernels: Go float semantics, the Go/Cephes
+// sine, and the clamped bilinear sample of the reference. Everything here is compiled
+// with -ffp-contract=off so each float operation rounds exactly as Go's float32 does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fl {
+
+// Grid description handed to every step kernel by value.
+struct Grid {
+  int NX, NY;     // NumX, NumY (width+2, height+2)
+  float h;        // cell size
+  float h1;       // float32(1.0/h)          fluid.go:360
+  float hs;       // float32(0.5*h)          fluid.go:361 (sample offset)
+  float ha;       // h/2                     fluid.go:298 (advect face offset)
+  float xmax;     // float32(NumX)*h         fluid.go:363
+  float ymax;     // float32(NumY)*h         fluid.go:364
+};
+
+// Cell flag bits (uint8 per cell), derived from S (S must be exactly 0.0 or 1.0).
+enum : uint32_t {
+  CF_FLUID = 1u << 0,   // S[i,j] != 0
+  CF_X0 = 1u << 1,      // S[i-1,j] != 0
+  CF_X1 = 1u << 2,      // S[i+1,j] != 0
+  CF_Y0 = 1u << 3,      // S[i,j-1] != 0
+  CF_Y1 = 1u << 4,      // S[i,j+1] != 0
+  CF_GS = 1u << 5,      // interior, fluid, and at least one fluid neighbour: updated by a GS sweep
+  CF_INTERIOR = 1u << 6 // 1 <= i <= NX-2 and 1 <= j <= NY-2
+};
+
+// Go builtin min/max on float32: NaN propagates, min(-0,+0) = -0, max(-0,+0) = +0.
+__device__ __forceinline__ float go_minf(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  if (a < b) return a;
+  if (b < a) return b;
+  return __builtin_signbit(a) ? a : b;
+}
+__device__ __forceinline__ float go_maxf(float a, float b) {
+  if (a != a) return a;
+  if (b != b) return b;
+  if (a > b) return a;
+  if (b > a) return b;
+  return __builtin_signbit(a) ? b : a;
+}
+
+// Go math.Sin for 0 <= |x| < 2^29 (Cephes sin.c octant reduction and polynomials),
+// evaluated in double without contraction.
+__device__ __forceinline__ double go_sin(double x) {
+  if (x == 0.0 || x != x) return x;
+  bool sign = false;
+  if (x < 0) {
+    x = -x;
+    sign = true;
+  }
+  const double PI4A = 7.85398125648498535156e-1;
+  const double PI4B = 3.77489470793079817668e-8;
+  const double PI4C = 2.69515142907905952645e-15;
+  uint64_t j = (uint64_t)(x * 0x1.45f306dc9c883p+0);
+  double y = (double)j;
+  if (j & 1) {
+    j++;
+    y += 1.0;
+  }
+  j &= 7;
+  double z = ((x - y * PI4A) - y * PI4B) - y * PI4C;
+  if (j > 3) {
+    sign = !sign;
+    j -= 4;
+  }
+  double zz = z * z;
+  if (j == 1 || j == 2) {
+    double c = -1.13585365213876817300e-11;
+    c = c * zz + 2.08757008419747316778e-9;
+    c = c * zz + -2.75573141792967388112e-7;
+    c = c * zz + 2.48015872888517045348e-5;
+    c = c * zz + -1.38888888888730564116e-3;
+    c = c * zz + 4.16666666666665929218e-2;
+    y = 1.0 - 0.5 * zz + zz * zz * c;
+  } else {
+    double s = 1.58962301576546568060e-10;
+    s = s * zz + -2.50507477628578072866e-8;
+    s = s * zz + 2.75573136213857245213e-6;
+    s = s * zz + -1.98412698295895385996e-4;
+    s = s * zz + 8.33333333332211858878e-3;
+    s = s * zz + -1.66666666666666307295e-1;
+    y = z + z * zz * s;
+  }
+  return sign ? -y : y;
+}
+
+// float32(math.Sin(float64(float32(k) * 0.01)))   (fluid.go:509-514)
+__device__ __forceinline__ float turb_noise(int k) {
+  float seed = (float)k * 0.01f;
+  return (float)go_sin((double)seed);
+}
+
+// int(math.Floor(float64(v))) with NaN/overflow mapped to 0 and flagged (the
+// reference would panic on the resulting slice index).
+__device__ __forceinline__ int floor_index(float v, int* bad) {
+  float f = floorf(v);
+  if (!(f >= -2147483648.0f && f < 2147483648.0f)) {
+    *bad = 1;
+    return 0;
+  }
+  return (int)f;
+}
+
+// sampleField / sampleFieldFrom (fluid.go:357-398, 1055-1091). (dx,dy) is (0,hs) for
+// U, (hs,0) for V and (hs,hs) for M. Evaluation order matches the reference exactly.
+__device__ __forceinline__ float sample_field(float x, float y, const float* __restrict__ f, float dx, float dy,
+                                              const Grid& g, int* bad) {
+  x = go_maxf(go_minf(x, g.xmax), g.h);
+  y = go_maxf(go_minf(y, g.ymax), g.h);
+  int x0 = min(floor_index((x - dx) * g.h1, bad), g.NX - 1);
+  float tx = ((x - dx) - (float)x0 * g.h) * g.h1;
+  int x1 = min(x0 + 1, g.NX - 1);
+  int y0 = min(floor_index((y - dy) * g.h1, bad), g.NY - 1);
+  float ty = ((y - dy) - (float)y0 * g.h) * g.h1;
+  int y1 = min(y0 + 1, g.NY - 1);
+  if (x0 < 0 || y0 < 0) {
+    *bad = 1;
+    x0 = max(x0, 0);
+    y0 = max(y0, 0);
+    x1 = max(x1, 0);
+    y1 = max(y1, 0);
+  }
+  float sx = 1.0f - tx;
+  float sy = 1.0f - ty;
+  const int n = g.NY;
+  float a = f[x0 * n + y0], b = f[x1 * n + y0], c = f[x1 * n + y1], d = f[x0 * n + y1];
+  return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
+}
+
+}  // namespace fl

@@ -1,0 +1,148 @@
+// Internal declarations of libfluidhip: the context that owns the device state of one
+// simulation, the host-side edit queue, and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fluid_hip.h"
+#include "fl_device.h"
+
+namespace fl {
+
+void set_error(const std::string& msg);
+
+#define FL_HIP(expr)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::fl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+      return FL_ERR_HIP;                                                                \
+    }                                                                                   \
+  } while (0)
+
+// One word-level edit: set or add `val` at field[idx]. Host API calls (SetSolid,
+// SetVelocity, AddSmoke, ApplyForce) expand to these, in call order.
+struct WordOp {
+  uint32_t key;   // field * numCells + idx
+  uint32_t kind;  // 0 = set, 1 = add
+  float val;
+};
+
+// Per-step source program (jet, sources, sinks): ops grouped per cell, in the
+// order the reference applies them (sources, sinks, jet; main.go:231-241,474-486).
+struct SrcOp {
+  int32_t cell;
+  int32_t kind;  // 0 = SetVelocity(u,v), 1 = AddSmoke(a)
+  int32_t check_solid;
+  float a, b;
+};
+
+struct Source {
+  int I, J;
+  float U, V;
+};
+
+// Pressure workspace of the temporal-blocked Gauss-Seidel kernel.
+struct GsWork {
+  int tk = 0;              // sweeps per tile (waves per workgroup)
+  int n_alloc = -1;        // sweep count the layout was built for
+  int G = 0, KB = 0;       // tiles along skewed rows and along sweeps
+  int rows = 0;            // handoff rows per tile (= steps per tile)
+  int ntiles = 0;          // non-empty tiles in ticket order
+  int grid = 0;            // persistent workgroups launched
+  int32_t* d_tile = nullptr;    // [ntiles] packed (g << 16 | kb), ticket order
+  float4* d_kout = nullptr;     // [G*KB][rows][65]
+  float4* d_gout = nullptr;     // [G*KB][rows][tk]
+  int32_t* d_prog = nullptr;    // [2][G*KB] progress counters (main run, rerun)
+  int32_t* d_ticket = nullptr;  // [2]
+  size_t kout_elems = 0, gout_elems = 0;
+};
+
+struct Ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int NX = 0, NY = 0;
+  size_t N = 0;  // NX*NY
+  float density = 0, h = 0;
+  fl_params prm{};
+  Grid grid{};
+
+  // Device state: the 8 reference buffers plus ping-pong partners and scratch.
+  float *U = nullptr, *V = nullptr, *nU = nullptr, *nV = nullptr, *p = nullptr, *S = nullptr, *M = nullptr,
+        *nM = nullptr;
+  float *U2 = nullptr, *V2 = nullptr, *M2 = nullptr;
+  float *T0 = nullptr, *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;  // BFECC / curl scratch
+  uint8_t* flags = nullptr;      // cell flags, i-major
+  uint8_t* flags_diag = nullptr; // cell flags, anti-diagonal-major [d=i+j][i]
+  bool flags_dirty = true;
+
+  // Solver control block on the device.
+  int32_t* d_ctrl = nullptr;       // [0] sweeps to run, [1] rerun sweeps, [2] invalid sample flag, [3] gs error
+  uint32_t* d_maxdiv = nullptr;    // [2][nmax] float bits of per-sweep max |div| (main, rerun)
+  float* d_omega = nullptr;        // [nmax] relaxation of each sweep
+  int nmax = 0;
+  uint32_t* d_red = nullptr;       // scratch for reductions
+  float* d_xy = nullptr;           // scratch for batched sampling
+  int xy_cap = 0;
+
+  GsWork gs;
+
+  // Host mirror of S (always valid) and pending edits.
+  std::vector<float> hS;
+  std::vector<WordOp> edits;
+  uint32_t* d_edit_keys = nullptr;
+  float* d_edit_vals = nullptr;
+  int32_t* d_edit_seg = nullptr;
+  size_t edit_cap = 0;
+
+  // Scene (package main) state.
+  bool wallTop = true, wallBottom = true, wallLeft = true, wallRight = false;
+  bool jet = false;
+  int jetX = 1, jetLo = 0, jetHi = 0;
+  float jetU = 4.0f, jetSmoke = 1.0f;
+  std::vector<Source> sources, sinks;
+  bool src_dirty = true;
+  SrcOp* d_src = nullptr;
+  int32_t* d_src_seg = nullptr;
+  int n_src_ops = 0, n_src_seg = 0;
+  size_t src_cap = 0;
+
+  // Stats of the last solve.
+  int last_n = 0;
+  bool last_rollback_checked = true;
+  int host_sweeps = -1;  // sweeps known on the host (hyperplane path), -1 if on device only
+};
+
+// ---- launchers (fl_kernels.hip) ------------------------------------------------------
+int launch_fill(float* dst, float v, size_t n, hipStream_t s);
+int launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
+int launch_flags(Ctx* c);
+int launch_copy_border(Ctx* c, float* dst, const float* src);
+int launch_viscosity(Ctx* c, float dt);
+int launch_curl(Ctx* c, float* curl);
+int launch_conf_turb(Ctx* c, const float* curl, float dt, int do_conf, int do_turb);
+int launch_borders(Ctx* c);
+// SL velocity advection: trace with (Ut,Vt), sample (Us,Vs); write nU/nV and (Uo,Vo).
+int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt);
+// BFECC backward pass fused with the correction and the clamp.
+int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd,
+                               const float* Vfwd, float* Ucorr, float* Vcorr, float dt);
+int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt);
+int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt);
+int launch_apply_edits(Ctx* c, int nseg);
+int launch_apply_sources(Ctx* c);
+int launch_max_reduce(Ctx* c, int kind, float* out);  // 0 MaxDivergence, 1 max(|U|+|V|)
+int launch_view(Ctx* c, int kind, float* out);        // 0 vorticity, 1 velocity magnitude
+int launch_sample_batch(Ctx* c, const float* xy, int n, float* uv);
+
+// ---- pressure (fl_pressure.hip) -------------------------------------------------------
+// Runs the single-grid projection: sweeps with the reference order on (U,V,p).
+// On return the projected velocity is in (c->U, c->V) and the pressure in c->p.
+int pressure_solve(Ctx* c, int numIters, float dt);
+int pressure_free(Ctx* c);
+
+}  // namespace fl

@@ -1,0 +1,315 @@
+"""Python mirror of the Go API of github.com/TheFellow/fluid/pkg/fluid over libfluidhip.
+
+Method and field names follow the reference (New, Simulate, SetSolid, IsSolid,
+SetVelocity, AddSmoke, Reset, ApplyForce, ApplyForceRadius, SetCircularObstacle,
+GetAdaptiveTimeStep, MaxDivergence, SampleVelocity, Vorticity, VelocityMagnitude,
+Pressure, Smoke, Velocity; knobs Confinement ... UseBFECC; package var Relaxation).
+Out-of-range indices raise IndexError with the reference's panic text
+(walls.go:6-11). The scene helpers restate package main's presets, jet, sources and
+strokes (main/main.go), which the reference keeps outside pkg/fluid.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import FlParams, FlStats, check
+
+FIELDS = {"U": 0, "V": 1, "newU": 2, "newV": 3, "p": 4, "S": 5, "M": 6, "newM": 7}
+PASSES = {
+    "copyBorderUV": 0,
+    "applyViscosity": 1,
+    "makeIncompressible": 2,
+    "applyVorticityConfinement": 3,
+    "addTurbulence": 4,
+    "handleBorders": 5,
+    "advectVelocity": 6,
+    "advectSmoke": 7,
+    "advectVelocityBFECC": 8,
+    "advectSmokeBFECC": 9,
+    "clearP": 10,
+}
+PRESETS = {"jet": 0, "cavity": 1, "karman": 2}
+
+# Package variable `Relaxation` (fluid.go:7-9): read by every Simulate.
+Relaxation = 1.9
+
+
+class ScalarField:
+    """ScalarField (scalar_field.go:7-22)."""
+
+    def __init__(self, nx, ny, values, mn, mx):
+        self.NumX, self.NumY = nx, ny
+        self.values = values
+        self.MinValue, self.MaxValue = mn, mx
+
+    def Value(self, i, j):
+        if i < 0 or i >= self.NumX:
+            return 0.0, ValueError("x index (%d) out of range, must be between 0 and %d" % (i, self.NumX - 1))
+        if j < 0 or j >= self.NumY:
+            return 0.0, ValueError("y index (%d) out of range, must be between 0 and %d" % (j, self.NumY - 1))
+        return float(self.values.reshape(-1)[i * self.NumY + j]), None
+
+
+class VectorField:
+    """VectorField (vector_field.go:5-20)."""
+
+    def __init__(self, nx, ny, u, v):
+        self.NumX, self.NumY = nx, ny
+        self.valuesU, self.valuesV = u, v
+
+    def Value(self, i, j):
+        if i < 0 or i >= self.NumX:
+            return 0.0, 0.0, ValueError("x index out of range, must be between 0 and %d" % (self.NumX - 1))
+        if j < 0 or j >= self.NumY:
+            return 0.0, 0.0, ValueError("y index out of range, must be between 0 and %d" % (self.NumY - 1))
+        k = i * self.NumY + j
+        return float(self.valuesU.reshape(-1)[k]), float(self.valuesV.reshape(-1)[k]), None
+
+
+class Fluid:
+    """Fluid (fluid.go:11-40) backed by device buffers on one MI355X."""
+
+    def __init__(self, density, width, height, h):
+        self._L = _lib.load()
+        ctx = ctypes.c_void_p()
+        check(self._L.fl_create(density, width, height, h, ctypes.byref(ctx)))
+        self._ctx = ctx
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        check(self._L.fl_dims(ctx, ctypes.byref(nx), ctypes.byref(ny)))
+        self.NumX, self.NumY = nx.value, ny.value
+        self._hval = float(np.float32(h))
+        self.Confinement = 0.0
+        self.ViscosityDiffusion = 0.0
+        self.PressureDamping = 1.0
+        self.TurbulenceStrength = 0.02
+        self.SmokeAdvection = 1.0
+        self.UseMultigrid = False
+        self.MultigridLevels = 2
+        self.UseBFECC = False
+        # Additions: the reference hard-codes 8 sweeps and the early exit (fluid.go:81,175).
+        self.PressureIters = 8
+        self.EarlyExit = True
+        self.Relaxation = None  # None: follow the module-level Relaxation like the Go package var
+        self.Solver = 0         # 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel
+        self.BlockSweeps = 0
+
+    @classmethod
+    def New(cls, density, width, height, h):
+        """New (fluid.go:42-68)."""
+        return cls(density, width, height, h)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.fl_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def H(self):
+        return self._hval
+
+    def _push(self):
+        p = FlParams()
+        p.confinement = self.Confinement
+        p.viscosity_diffusion = self.ViscosityDiffusion
+        p.pressure_damping = self.PressureDamping
+        p.turbulence_strength = self.TurbulenceStrength
+        p.smoke_advection = self.SmokeAdvection
+        p.relaxation = Relaxation if self.Relaxation is None else self.Relaxation
+        p.use_multigrid = int(bool(self.UseMultigrid))
+        p.multigrid_levels = int(self.MultigridLevels)
+        p.use_bfecc = int(bool(self.UseBFECC))
+        p.pressure_iters = int(self.PressureIters)
+        p.early_exit = int(bool(self.EarlyExit))
+        p.solver = int(self.Solver)
+        p.block_sweeps = int(self.BlockSweeps)
+        check(self._L.fl_set_params(self._ctx, ctypes.byref(p)))
+
+    # ---- whole-field access ---------------------------------------------------------
+    def get(self, name):
+        a = np.empty((self.NumX, self.NumY), dtype=np.float32)
+        check(self._L.fl_download(self._ctx, FIELDS[name], a.ctypes.data, a.size))
+        return a
+
+    def set(self, name, arr):
+        a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).reshape(self.NumX, self.NumY))
+        check(self._L.fl_upload(self._ctx, FIELDS[name], a.ctypes.data, a.size))
+
+    def state(self):
+        return {k: self.get(k) for k in FIELDS}
+
+    def load_state(self, st):
+        for k, v in st.items():
+            self.set(k, v)
+
+    # The exported slices U, V, S, M (fluid.go:17-22) as downloaded copies.
+    U = property(lambda self: self.get("U"))
+    V = property(lambda self: self.get("V"))
+    S = property(lambda self: self.get("S"))
+    M = property(lambda self: self.get("M"))
+
+    # ---- step -------------------------------------------------------------------------
+    def Simulate(self, dt):
+        """Simulate (fluid.go:79-109)."""
+        self._push()
+        check(self._L.fl_simulate(self._ctx, dt))
+
+    def sync(self):
+        check(self._L.fl_sync(self._ctx))
+
+    def stats(self):
+        s = FlStats()
+        check(self._L.fl_get_stats(self._ctx, ctypes.byref(s)))
+        return {"sweeps_run": s.sweeps_run, "reran": bool(s.reran), "invalid_sample": bool(s.invalid_sample),
+                "max_div_last": s.max_div_last}
+
+    def sweep_maxdiv(self):
+        cnt = ctypes.c_int()
+        check(self._L.fl_sweep_maxdiv(self._ctx, None, 0, ctypes.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=np.float32)
+        if cnt.value:
+            check(self._L.fl_sweep_maxdiv(self._ctx, out.ctypes.data, cnt.value, ctypes.byref(cnt)))
+        return out
+
+    def run_pass(self, name, dt=0.0, iters=0):
+        """Run one unexported pass (copyBorder, makeIncompressible, advectVelocity, ...)."""
+        self._push()
+        check(self._L.fl_pass(self._ctx, PASSES[name], dt, iters))
+
+    # ---- walls.go -----------------------------------------------------------------------
+    def _idx(self, rc):
+        if rc in (2, 3):
+            raise IndexError(self._L.fl_last_error().decode())
+        check(rc)
+
+    def SetSolid(self, i, j, value):
+        self._idx(self._L.fl_set_solid(self._ctx, i, j, int(bool(value))))
+
+    def IsSolid(self, i, j):
+        out = ctypes.c_int()
+        self._idx(self._L.fl_is_solid(self._ctx, i, j, ctypes.byref(out)))
+        return bool(out.value)
+
+    def SetVelocity(self, i, j, u, v):
+        self._idx(self._L.fl_set_velocity(self._ctx, i, j, u, v))
+
+    def AddSmoke(self, i, j, s):
+        self._idx(self._L.fl_add_smoke(self._ctx, i, j, s))
+
+    def Reset(self):
+        check(self._L.fl_reset(self._ctx))
+
+    # ---- fluid.go API ---------------------------------------------------------------------
+    def ApplyForce(self, i, j, fx, fy):
+        check(self._L.fl_apply_force(self._ctx, i, j, fx, fy))
+
+    def ApplyForceRadius(self, cx, cy, fx, fy, radius):
+        check(self._L.fl_apply_force_radius(self._ctx, cx, cy, fx, fy, radius))
+
+    def SetCircularObstacle(self, cx, cy, radius):
+        check(self._L.fl_set_circular_obstacle(self._ctx, cx, cy, radius))
+
+    def GetAdaptiveTimeStep(self, basedt):
+        out = ctypes.c_float()
+        self._push()
+        check(self._L.fl_get_adaptive_time_step(self._ctx, basedt, ctypes.byref(out)))
+        return out.value
+
+    def MaxDivergence(self):
+        out = ctypes.c_float()
+        check(self._L.fl_max_divergence(self._ctx, ctypes.byref(out)))
+        return out.value
+
+    def SampleVelocity(self, x, y):
+        u, v = ctypes.c_float(), ctypes.c_float()
+        check(self._L.fl_sample_velocity(self._ctx, x, y, ctypes.byref(u), ctypes.byref(v)))
+        return u.value, v.value
+
+    def SampleVelocityBatch(self, xs, ys):
+        xs = np.ascontiguousarray(xs, dtype=np.float32)
+        ys = np.ascontiguousarray(ys, dtype=np.float32)
+        us, vs = np.empty_like(xs), np.empty_like(ys)
+        check(self._L.fl_sample_velocity_batch(self._ctx, xs.ctypes.data, ys.ctypes.data, xs.size, us.ctypes.data,
+                                               vs.ctypes.data))
+        return us, vs
+
+    def _view(self, fn):
+        out = np.empty((self.NumX, self.NumY), dtype=np.float32)
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        check(fn(self._ctx, out.ctypes.data, out.size, ctypes.byref(mn), ctypes.byref(mx)))
+        return ScalarField(self.NumX, self.NumY, out, mn.value, mx.value)
+
+    def Vorticity(self):
+        return self._view(self._L.fl_vorticity)
+
+    def VelocityMagnitude(self):
+        return self._view(self._L.fl_velocity_magnitude)
+
+    def _minmax(self, name):
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        check(self._L.fl_field_minmax(self._ctx, FIELDS[name], ctypes.byref(mn), ctypes.byref(mx)))
+        return ScalarField(self.NumX, self.NumY, self.get(name), mn.value, mx.value)
+
+    def Pressure(self):
+        return self._minmax("p")
+
+    def Smoke(self):
+        return self._minmax("M")
+
+    def Velocity(self):
+        return VectorField(self.NumX, self.NumY, self.get("U"), self.get("V"))
+
+    # ---- scene layer (package main) ----------------------------------------------------------
+    def scene_walls(self, top, bottom, left, right):
+        check(self._L.fl_scene_set_walls(self._ctx, int(top), int(bottom), int(left), int(right)))
+
+    def apply_wall_settings(self):
+        check(self._L.fl_scene_apply_walls(self._ctx))
+
+    def clear_walls(self):
+        check(self._L.fl_scene_clear_walls(self._ctx))
+
+    def reset_to_preset(self, preset, lid_u=3.0):
+        check(self._L.fl_scene_preset(self._ctx, PRESETS.get(preset, preset), lid_u))
+
+    def set_jet(self, on, x=1, lo=0, hi=0, u=4.0, smoke=1.0):
+        self._idx(self._L.fl_scene_set_jet(self._ctx, int(bool(on)), x, lo, hi, u, smoke))
+
+    def add_source(self, i, j, u, v):
+        check(self._L.fl_scene_add_source(self._ctx, 0, i, j, u, v))
+
+    def add_sink(self, i, j, u, v):
+        check(self._L.fl_scene_add_source(self._ctx, 1, i, j, u, v))
+
+    def clear_sources(self):
+        check(self._L.fl_scene_clear_sources(self._ctx))
+
+    def apply_sources(self):
+        check(self._L.fl_scene_apply_sources(self._ctx))
+
+    def update(self, dt):
+        """Game.Update minus input and particles (main.go:231-249): sources, jet, Simulate."""
+        self._push()
+        check(self._L.fl_scene_update(self._ctx, dt))
+
+    def draw_line(self, x0, y0, x1, y1, value):
+        check(self._L.fl_scene_draw_line(self._ctx, x0, y0, x1, y1, int(bool(value))))
+
+    def smoke_brush(self, ci, cj, r, strength):
+        check(self._L.fl_scene_smoke_brush(self._ctx, ci, cj, r, strength))
+
+
+def New(density, width, height, h):
+    """New (fluid.go:42)."""
+    return Fluid(density, width, height, h)
+
+
+def device_count():
+    n = ctypes.c_int()
+    _lib.load().fl_device_count(ctypes.byref(n))
+    return n.value

@@ -1,0 +1,167 @@
+/*
+ * fluid_hip.h — C ABI of libfluidhip.so, the MI355X (gfx950) implementation of the
+ * TheFellow/fluid `pkg/fluid` simulation step.
+ *
+ * The reference has no FFI: its interface is the Go package API
+ * (github.com/TheFellow/fluid/pkg/fluid). Each entry point below replaces one Go
+ * function or method; the reference file:line it stands for is given beside it.
+ * A cgo binding that restores the exact Go API on top of these calls is shown in
+ * INTEGRATION.md. Plain pointers and sizes only; no Go pointer is retained.
+ *
+ * Layout: every field is float32 [NumX*NumY] with index i*NumY + j (j contiguous),
+ * NumX = width+2, NumY = height+2, as in the reference (fluid.go:42-58).
+ *
+ * Error behaviour: every call returns an fl_status. Where the reference panics
+ * (walls.go:6-11 and its siblings) the call returns FL_ERR_X_RANGE / FL_ERR_Y_RANGE
+ * and fl_last_error() carries the reference's panic text. ApplyForce ignores
+ * out-of-range and solid cells like the reference (fluid.go:762-768).
+ *
+ * Threading: a context is not safe for concurrent use (the reference Fluid is not
+ * either); it owns one HIP stream. Calls on one context are ordered.
+ */
+#ifndef FLUID_HIP_H
+#define FLUID_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fl_ctx fl_ctx;
+
+typedef enum fl_status {
+  FL_OK = 0,
+  FL_ERR_INVALID_ARG = 1,
+  FL_ERR_X_RANGE = 2,     /* reference: panic("invalid x-index: %d") */
+  FL_ERR_Y_RANGE = 3,     /* reference: panic("invalid y-index: %d") */
+  FL_ERR_NO_DEVICE = 4,
+  FL_ERR_HIP = 5,
+  FL_ERR_UNSUPPORTED = 6,
+  FL_ERR_INTERNAL = 7
+} fl_status;
+
+typedef enum fl_field {
+  FL_FIELD_U = 0,    /* Fluid.U    (fluid.go:17) */
+  FL_FIELD_V = 1,    /* Fluid.V    (fluid.go:17) */
+  FL_FIELD_NEWU = 2, /* Fluid.newU (fluid.go:18), stale-value semantics kept */
+  FL_FIELD_NEWV = 3, /* Fluid.newV (fluid.go:18) */
+  FL_FIELD_P = 4,    /* Fluid.p    (fluid.go:19) */
+  FL_FIELD_S = 5,    /* Fluid.S    (fluid.go:20); values must be exactly 0.0 or 1.0 */
+  FL_FIELD_M = 6,    /* Fluid.M    (fluid.go:22) */
+  FL_FIELD_NEWM = 7  /* Fluid.newM (fluid.go:23) */
+} fl_field;
+
+/* Exported knobs of Fluid (fluid.go:25-39) plus the package variable Relaxation
+ * (fluid.go:7-9). pressure_iters and early_exit are additions: the reference
+ * hard-codes 8 sweeps (fluid.go:81) with the maxDiv<1e-5 exit (fluid.go:175-177). */
+typedef struct fl_params {
+  float confinement;         /* Confinement, default 0 */
+  float viscosity_diffusion; /* ViscosityDiffusion, default 0 */
+  float pressure_damping;    /* PressureDamping, default 1 */
+  float turbulence_strength; /* TurbulenceStrength, default 0.02 */
+  float smoke_advection;     /* SmokeAdvection, default 1 */
+  float relaxation;          /* Relaxation (package var), default 1.9 */
+  int32_t use_multigrid;     /* UseMultigrid, default 0 */
+  int32_t multigrid_levels;  /* MultigridLevels, default 2 */
+  int32_t use_bfecc;         /* UseBFECC, default 0 */
+  int32_t pressure_iters;    /* sweeps per Simulate, default 8 */
+  int32_t early_exit;        /* 1: stop after the first sweep with maxDiv < 1e-5 */
+  int32_t solver;            /* 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel */
+  int32_t block_sweeps;      /* sweeps per temporal block (0 = tuned default) */
+} fl_params;
+
+/* Statistics of the last Simulate (no reference counterpart). */
+typedef struct fl_stats {
+  int32_t sweeps_run;     /* pressure sweeps whose result was kept */
+  int32_t reran;          /* 1 if the early exit rolled back and re-ran fewer sweeps */
+  int32_t invalid_sample; /* 1 if a NaN/Inf reached a sample index (the reference panics) */
+  float max_div_last;     /* maxDiv of the last kept sweep */
+} fl_stats;
+
+const char* fl_last_error(void);
+int fl_device_count(int* n);
+
+/* New(density, width, height, h) — fluid.go:42-68. S starts all zero (all solid). */
+int fl_create(float density, int width, int height, float h, fl_ctx** out);
+int fl_destroy(fl_ctx* ctx);
+/* NumX, NumY (fluid.go:15) and H() (fluid.go:71). */
+int fl_dims(fl_ctx* ctx, int* num_x, int* num_y);
+int fl_get_h(fl_ctx* ctx, float* h);
+int fl_get_params(fl_ctx* ctx, fl_params* out);
+int fl_set_params(fl_ctx* ctx, const fl_params* in);
+
+/* Whole-field access to the exported slices U, V, S, M and the internal buffers
+ * (the reference exposes U,V,S,M as mutable slices, fluid.go:17-22). n = NumX*NumY. */
+int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n);
+int fl_download(fl_ctx* ctx, int field, float* dst, size_t n);
+
+/* Simulate(dt) — fluid.go:79-109. Asynchronous on the context's stream. */
+int fl_simulate(fl_ctx* ctx, float dt);
+int fl_sync(fl_ctx* ctx);
+int fl_get_stats(fl_ctx* ctx, fl_stats* out);
+/* maxDiv of every sweep of the last solve (speculative run, before a rollback). */
+int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count);
+
+/* walls.go:5-93 */
+int fl_set_solid(fl_ctx* ctx, int i, int j, int solid);   /* SetSolid  walls.go:5  */
+int fl_is_solid(fl_ctx* ctx, int i, int j, int* out);      /* IsSolid   walls.go:51 */
+int fl_set_velocity(fl_ctx* ctx, int i, int j, float u, float v); /* walls.go:62 */
+int fl_add_smoke(fl_ctx* ctx, int i, int j, float s);      /* AddSmoke  walls.go:74 */
+int fl_reset(fl_ctx* ctx);                                 /* Reset     walls.go:85 */
+
+/* fluid.go:761-907 */
+int fl_apply_force(fl_ctx* ctx, int i, int j, float fx, float fy);            /* :761 */
+int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int radius); /* :774 */
+int fl_set_circular_obstacle(fl_ctx* ctx, int cx, int cy, int radius);        /* :894 */
+int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out);         /* :529 */
+int fl_max_divergence(fl_ctx* ctx, float* out);                               /* :876 */
+int fl_sample_velocity(fl_ctx* ctx, float x, float y, float* u, float* v);    /* :799 */
+/* Batched SampleVelocity for tracer particles (main.go:527-530 calls it per particle). */
+int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int n, float* us, float* vs);
+/* Vorticity() (:806) and VelocityMagnitude() (:841): the field plus MinValue/MaxValue. */
+int fl_vorticity(fl_ctx* ctx, float* out, size_t n, float* min_value, float* max_value);
+int fl_velocity_magnitude(fl_ctx* ctx, float* out, size_t n, float* min_value, float* max_value);
+/* Pressure() (pressure.go:5) / Smoke() (smoke.go:5): MinValue/MaxValue over all cells. */
+int fl_field_minmax(fl_ctx* ctx, int field, float* min_value, float* max_value);
+
+/* Unexported passes, for white-box tests in the manner of fluid_test.go. */
+typedef enum fl_pass_id {
+  FL_PASS_COPY_BORDER_UV = 0,   /* copyBorder(newU,U); copyBorder(newV,V)  fluid.go:436 */
+  FL_PASS_VISCOSITY = 1,        /* applyViscosity          fluid.go:112 */
+  FL_PASS_MAKE_INCOMPRESSIBLE = 2, /* makeIncompressible(iters,dt) fluid.go:144 */
+  FL_PASS_CONFINEMENT = 3,      /* applyVorticityConfinement fluid.go:449 */
+  FL_PASS_TURBULENCE = 4,       /* addTurbulence           fluid.go:496 */
+  FL_PASS_BORDERS = 5,          /* handleBorders           fluid.go:236 */
+  FL_PASS_ADVECT_VELOCITY = 6,  /* advectVelocity          fluid.go:291 */
+  FL_PASS_ADVECT_SMOKE = 7,     /* advectSmoke             fluid.go:400 */
+  FL_PASS_ADVECT_VELOCITY_BFECC = 8, /* advectVelocityBFECC  fluid.go:911 */
+  FL_PASS_ADVECT_SMOKE_BFECC = 9,    /* advectSmokeBFECC     fluid.go:997 */
+  FL_PASS_CLEAR_P = 10          /* fill(p, 0)              fluid.go:83 */
+} fl_pass_id;
+int fl_pass(fl_ctx* ctx, int pass, float dt, int iters);
+
+/* Scene layer lifted from package main (main/main.go), which the reference keeps
+ * outside pkg/fluid. Presets: 0 jet, 1 cavity, 2 karman (main.go:747-793). */
+int fl_scene_set_walls(fl_ctx* ctx, int top, int bottom, int left, int right);
+int fl_scene_apply_walls(fl_ctx* ctx);   /* applyWallSettings main.go:840 */
+int fl_scene_clear_walls(fl_ctx* ctx);   /* clearWalls        main.go:851 */
+int fl_scene_preset(fl_ctx* ctx, int preset, float lid_u); /* resetToPreset main.go:747 (lid_u=3 there) */
+/* Jet (main.go:236-241): each Update does SetVelocity(x,j,u,0); AddSmoke(x,j,smoke)
+ * for j in [j_lo, j_hi). */
+int fl_scene_set_jet(fl_ctx* ctx, int on, int x, int j_lo, int j_hi, float u, float smoke);
+/* Sources (sink=0) and sinks (sink=1), main.go:147-150,474-486. */
+int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v);
+int fl_scene_clear_sources(fl_ctx* ctx);
+int fl_scene_apply_sources(fl_ctx* ctx); /* applySources main.go:474 */
+/* Game.Update without input and particles (main.go:231-249): sources, jet, Simulate. */
+int fl_scene_update(fl_ctx* ctx, float dt);
+int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid); /* drawLine main.go:809 */
+int fl_scene_smoke_brush(fl_ctx* ctx, int ci, int cj, int radius, float strength); /* main.go:453 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUID_HIP_H */

@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Throughput of the pkg/fluid simulation step on one MI355X.
+
+Workload (default): the lid-driven cavity at 4096x4096, FP32, seeded solid discs over
+>= 5% of the interior, BFECC advection, 200 Gauss-Seidel pressure sweeps per step,
+dt 0.1. A "step" is one Update of the scene: per-step lid sources, then Simulate.
+
+Prints one JSON line: cell-steps/s over the timed steps, the roofline of the pressure
+solve (algorithmic bytes per solve / its average device time, against 8 TB/s HBM), and
+a CPU baseline (the float32 oracle on this host, a bounded sample of the same workload).
+
+Algorithmic bytes per interior cell-step (each pass reads each input field once and
+writes each output once, FP32 fields as in the reference): p clear 4, GS sweep 28 per
+sweep, confinement 40, turbulence 20, SL velocity 20, SL smoke 20, BFECC velocity 92,
+BFECC smoke 72, viscosity 20.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-steps/s & % HBM peak (FP32, 200 Jacobi) vs Go CPU; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_cell(iters, bfecc, conf, turb, visc):
+    b = 4 + 28 * iters
+    b += 40 if conf else 0
+    b += 20 if turb else 0
+    b += 20 if visc else 0
+    b += (92 + 72) if bfecc else (20 + 20)
+    return b
+
+
+def build(kind, make, size, iters):
+    from fluid_amd import configs as C
+    if kind == "cavity":
+        return C.cavity_scene(make, size, size, iters)
+    if kind == "jet":
+        return C.jet_scene(make, size, size, iters)
+    if kind == "karman":
+        return C.karman_scene(make, 2 * size, size, iters, radius=size // 16, centre=(size // 4, size // 2))
+    raise SystemExit("unknown config %s" % kind)
+
+
+def cpu_baseline(kind, size, iters, threads):
+    """The float32 oracle stepping the same scene on this host's cores: a bounded sample,
+    one step of the workload at the largest power-of-two size whose step is expected to
+    take under ~20 s (the serial Gauss-Seidel sweeps cost ~15 ns per cell-sweep)."""
+    from oracle.pyoracle import OracleFluid
+    sample_size = size
+    while sample_size > 256 and (sample_size ** 2) * iters * 15e-9 > 20.0:
+        sample_size //= 2
+    f = build(kind, lambda d, w, h, hh: OracleFluid(d, w, h, hh, threads=threads), sample_size, iters)
+    t0 = time.perf_counter()
+    f.update(0.1)
+    dt = time.perf_counter() - t0
+    cells = sample_size * sample_size * (2 if kind == "karman" else 1)
+    return {"value": cells / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": "float32 oracle (C++ restatement of pkg/fluid, parallelRange threading), one %s step "
+                      "at %dx%d, %.2f s" % (kind, sample_size * (2 if kind == "karman" else 1),
+                                                                   sample_size, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman"])
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--solver", type=int, default=0)
+    ap.add_argument("--block-sweeps", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU decomposition "
+              "is described in DESIGN.md and not built)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from fluid_amd import Fluid
+    from fluid_amd import configs as C
+
+    print("bench: building %s %d^2, %d sweeps" % (args.config, args.size, args.iters), file=sys.stderr, flush=True)
+    f = build(args.config, lambda *a: Fluid(*a), args.size, args.iters)
+    f.Solver = args.solver
+    f.BlockSweeps = args.block_sweeps
+    W, H = f.NumX - 2, f.NumY - 2
+    cells = W * H
+    for _ in range(args.warmup):
+        f.update(C.DT)
+    f.sync()
+    print("bench: warm-up done, timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f.update(C.DT)
+    f.sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = cells * args.steps / wall
+    st = f.stats()
+
+    # Per-pass device times from HIP events on the library's stream.
+    f.set_timing(True)
+    nt = max(1, min(args.steps, 5))
+    for _ in range(nt):
+        f.update(C.DT)
+    tm = f.timing()
+    f.set_timing(False)
+    solve_ms = tm["pressure_ms"] / tm["steps"]
+    sweeps = st["sweeps_run"]
+    solve_bytes = 28.0 * sweeps * cells
+    achieved = solve_bytes / (solve_ms * 1e-3) / 1e9
+    b_alg = alg_bytes_per_cell(sweeps, f.UseBFECC, f.Confinement != 0, f.TurbulenceStrength > 0,
+                               f.ViscosityDiffusion > 0)
+    step_gbs = b_alg * cells / (ms_per_step * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "%s %dx%d FP32, %d GS sweeps/step, BFECC %s, dt 0.1" % (
+            args.config, W, H, sweeps, "on" if f.UseBFECC else "off"), "grid": [W, H], "pressure_sweeps": sweeps,
+            "solver": ["auto", "hyperplane", "temporal-blocked"][args.solver]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},
+        "step_alg_bytes_per_cell": b_alg,
+        "step_achieved_GBs": step_gbs,
+        "step_frac_hbm": step_gbs / HBM_PEAK_GBS,
+        "device_ms_per_step": {"total": tm["total_ms"] / tm["steps"], "pressure": solve_ms,
+                               "advect": tm["advect_ms"] / tm["steps"], "other": tm["other_ms"] / tm["steps"]},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (float32 oracle, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.config, args.size, args.iters, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -41,6 +41,17 @@ class FlStats(ctypes.Structure):
     ]
 
 
+class FlTiming(ctypes.Structure):
+    _fields_ = [
+        ("total_ms", ctypes.c_double),
+        ("pressure_ms", ctypes.c_double),
+        ("advect_ms", ctypes.c_double),
+        ("other_ms", ctypes.c_double),
+        ("steps", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 # Every symbol declared in include/fluid_hip.h with its ctypes signature.
 _vp, _i, _f, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
 _ip, _fp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
@@ -59,6 +70,8 @@ SIGNATURES = {
     "fl_sync": (_i, [_vp]),
     "fl_get_stats": (_i, [_vp, ctypes.POINTER(FlStats)]),
     "fl_sweep_maxdiv": (_i, [_vp, _vp, _i, _ip]),
+    "fl_set_timing": (_i, [_vp, _i]),
+    "fl_get_timing": (_i, [_vp, ctypes.POINTER(FlTiming)]),
     "fl_set_solid": (_i, [_vp, _i, _i, _i]),
     "fl_is_solid": (_i, [_vp, _i, _i, _ip]),
     "fl_set_velocity": (_i, [_vp, _i, _i, _f, _f]),

@@ -201,24 +201,52 @@ static int advect_smoke_bfecc(Ctx* c, float dt) {
   return launch_advect_smoke(c, c->T2, c->M, dt);
 }
 
-static int simulate(Ctx* c, float dt) {
-  int rc = prepare(c);
-  if (rc) return rc;
-  if (c->NX < 3 || c->NY < 3) return err(FL_ERR_UNSUPPORTED, "grid must have at least 1x1 interior cells");
+static int mark(Ctx* c, int k) {
+  if (c->timing) FL_HIP(hipEventRecord(c->ev[k], c->stream));
+  return FL_OK;
+}
+
+static int simulate_passes(Ctx* c, float dt) {
+  int rc;
   const fl_params& P = c->prm;
   if ((rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;                      // fill(p,0)
   if (P.viscosity_diffusion > 0 && (rc = launch_viscosity(c, dt))) return rc;          // applyViscosity
+  if ((rc = mark(c, 1))) return rc;
   if ((rc = make_incompressible(c, P.pressure_iters, dt, false))) return rc;           // makeIncompressible
+  if ((rc = mark(c, 2))) return rc;
   const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
   if (conf && (rc = launch_curl(c, c->T0))) return rc;
   if ((conf || turb) && (rc = launch_conf_turb(c, c->T0, dt, conf, turb))) return rc;  // confinement, turbulence
   if ((rc = launch_borders(c))) return rc;                                             // handleBorders
+  if ((rc = mark(c, 3))) return rc;
   if (P.use_bfecc) {
     if ((rc = advect_velocity_bfecc(c, dt))) return rc;
     return advect_smoke_bfecc(c, dt);
   }
   if ((rc = advect_velocity_sl(c, dt))) return rc;
   return advect_smoke_sl(c, dt);
+}
+
+// Simulate (fluid.go:79-109). Event 0 is recorded by the caller before any per-step
+// source injection so that it is counted as part of the step.
+static int simulate(Ctx* c, float dt, bool start_marked = false) {
+  int rc = prepare(c);
+  if (rc) return rc;
+  if (c->NX < 3 || c->NY < 3) return err(FL_ERR_UNSUPPORTED, "grid must have at least 1x1 interior cells");
+  if (!start_marked && (rc = mark(c, 0))) return rc;
+  if ((rc = simulate_passes(c, dt))) return rc;
+  if (c->timing) {
+    if ((rc = mark(c, 4))) return rc;
+    FL_HIP(hipEventSynchronize(c->ev[4]));
+    float t[4];
+    for (int k = 0; k < 4; k++) FL_HIP(hipEventElapsedTime(&t[k], c->ev[k], c->ev[k + 1]));
+    c->t_other += t[0] + t[2];
+    c->t_pressure += t[1];
+    c->t_advect += t[3];
+    c->t_total += t[0] + t[1] + t[2] + t[3];
+    c->t_steps++;
+  }
+  return FL_OK;
 }
 
 static void free_ctx(Ctx* c) {
@@ -238,6 +266,8 @@ static void free_ctx(Ctx* c) {
   if (c->d_src) (void)hipFree(c->d_src);
   if (c->d_src_seg) (void)hipFree(c->d_src_seg);
   pressure_free(c);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -511,6 +541,29 @@ int fl_get_stats(fl_ctx* ctx, fl_stats* out) {
     std::memcpy(&out->max_div_last, &bits, 4);
   }
   if (ctrl[3] != 0) return err(FL_ERR_INTERNAL, "pressure kernel reported a dependency wait timeout");
+  return FL_OK;
+}
+
+int fl_set_timing(fl_ctx* ctx, int on) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  if (on && !c->ev[0])
+    for (auto& e : c->ev) FL_HIP(hipEventCreate(&e));
+  c->timing = on != 0;
+  c->t_total = c->t_pressure = c->t_advect = c->t_other = 0;
+  c->t_steps = 0;
+  return FL_OK;
+}
+
+int fl_get_timing(fl_ctx* ctx, fl_timing* out) {
+  if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
+  Ctx* c = C_OF(ctx);
+  *out = fl_timing{};
+  out->total_ms = c->t_total;
+  out->pressure_ms = c->t_pressure;
+  out->advect_ms = c->t_advect;
+  out->other_ms = c->t_other;
+  out->steps = c->t_steps;
   return FL_OK;
 }
 
@@ -821,8 +874,9 @@ int fl_scene_update(fl_ctx* ctx, float dt) {
   int rc = flush_edits(c);
   if (rc) return rc;
   if ((rc = build_sources(c))) return rc;
+  if ((rc = mark(c, 0))) return rc;
   if ((rc = launch_apply_sources(c))) return rc;
-  return simulate(c, dt);
+  return simulate(c, dt, true);
 }
 int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid) {
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");

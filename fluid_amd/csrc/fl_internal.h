@@ -111,6 +111,12 @@ struct Ctx {
   int n_src_ops = 0, n_src_seg = 0;
   size_t src_cap = 0;
 
+  // Optional per-pass timing (fl_set_timing).
+  bool timing = false;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  double t_total = 0, t_pressure = 0, t_advect = 0, t_other = 0;
+  int t_steps = 0;
+
   // Stats of the last solve.
   int last_n = 0;
   bool last_rollback_checked = true;

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import FlParams, FlStats, check
+from ._lib import FlParams, FlStats, FlTiming, check
 
 FIELDS = {"U": 0, "V": 1, "newU": 2, "newV": 3, "p": 4, "S": 5, "M": 6, "newM": 7}
 PASSES = {
@@ -167,6 +167,16 @@ class Fluid:
         check(self._L.fl_get_stats(self._ctx, ctypes.byref(s)))
         return {"sweeps_run": s.sweeps_run, "reran": bool(s.reran), "invalid_sample": bool(s.invalid_sample),
                 "max_div_last": s.max_div_last}
+
+    def set_timing(self, on):
+        """Per-pass device timing with HIP events on the context's stream."""
+        check(self._L.fl_set_timing(self._ctx, int(bool(on))))
+
+    def timing(self):
+        t = FlTiming()
+        check(self._L.fl_get_timing(self._ctx, ctypes.byref(t)))
+        return {"total_ms": t.total_ms, "pressure_ms": t.pressure_ms, "advect_ms": t.advect_ms,
+                "other_ms": t.other_ms, "steps": t.steps}
 
     def sweep_maxdiv(self):
         cnt = ctypes.c_int()

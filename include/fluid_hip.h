@@ -101,6 +101,19 @@ int fl_download(fl_ctx* ctx, int field, float* dst, size_t n);
 int fl_simulate(fl_ctx* ctx, float dt);
 int fl_sync(fl_ctx* ctx);
 int fl_get_stats(fl_ctx* ctx, fl_stats* out);
+/* Per-pass device time (no reference counterpart): when enabled, every Simulate records
+ * HIP events on the context's stream around its pass groups and waits for them. */
+typedef struct fl_timing {
+  double total_ms;     /* whole steps */
+  double pressure_ms;  /* pressure projection (all sweeps, rollback included) */
+  double advect_ms;    /* velocity + smoke advection (SL or BFECC) */
+  double other_ms;     /* p clear, viscosity, confinement, turbulence, borders, sources */
+  int32_t steps;
+  int32_t pad;
+} fl_timing;
+int fl_set_timing(fl_ctx* ctx, int on); /* also resets the accumulators */
+int fl_get_timing(fl_ctx* ctx, fl_timing* out);
+
 /* maxDiv of every sweep of the last solve (speculative run, before a rollback). */
 int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count);
 

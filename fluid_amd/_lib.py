@@ -7,7 +7,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfluidhip.so")
+LIB_PATH = os.environ.get("FLUID_AMD_LIB", os.path.join(HERE, "libfluidhip.so"))
 
 FL_OK = 0
 STATUS = {0: "FL_OK", 1: "FL_ERR_INVALID_ARG", 2: "FL_ERR_X_RANGE", 3: "FL_ERR_Y_RANGE", 4: "FL_ERR_NO_DEVICE",

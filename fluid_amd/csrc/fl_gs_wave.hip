@@ -15,8 +15,12 @@
 // initial U,V once, the final U,V,p once, and per tile and column the boundary faces
 // handed to the next tile along i' (G stream) and the next block of sweeps (K stream).
 // Tiles run as a persistent kernel in ticket order (a topological order of the tile
-// graph); a consumer waits on its producers' published progress (release/acquire at
-// agent scope), in blocks of 8 columns.
+// graph); a consumer waits on its producers' published progress, in blocks of 8 columns.
+// Handoff data and progress counters are accessed only with agent-scope relaxed atomic
+// loads and stores (device-coherent accesses, global_load/store sc1), so neither side
+// needs an L2 writeback or invalidate; the producer waits for its data stores to
+// complete (s_waitcnt vmcnt(0)) before it stores the new progress value, and consumers
+// issue their data loads only after observing it.
 //
 // Cells outside the interior or solid pass their faces through unchanged, exactly like
 // the skipped cells of the serial loop. Faces no interior cell touches keep their input
@@ -66,15 +70,29 @@ __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX)
   return rmax < 0 || rmin > NX - 1;
 }
 
+// Device-coherent (agent scope) accesses of the handoff buffers.
+__device__ __forceinline__ float ld_dev(const float* p) {
+  return __hip_atomic_load(const_cast<float*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(float* p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float4 ld_dev4(const float4* p) {
+  const float* f = reinterpret_cast<const float*>(p);
+  return make_float4(ld_dev(f), ld_dev(f + 1), ld_dev(f + 2), ld_dev(f + 3));
+}
+// Wait until every store this wave issued has been acknowledged by the memory system.
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ int wait_progress(int32_t* p, int need, int32_t* err) {
-  int v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int it = 0; v < need; it++) {
     if (it > SPIN_LIMIT) {
       atomicOr(err, 1);
       return need;
     }
     __builtin_amdgcn_s_sleep(2);
-    v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return v;
 }
@@ -109,7 +127,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
     const int g = A.tiles[t] >> 16, kb = A.tiles[t] & 0xffff;
     const int me = g * KB + kb;
     if (tile_empty(g, kb, K, TK, NX)) {
-      if (tid == 0) __hip_atomic_store(&A.prog[me], rows, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (tid == 0) __hip_atomic_store(&A.prog[me], rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       continue;
     }
     const int k0 = kb * TK;
@@ -173,13 +191,15 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           } else {
             if (pK >= 0) {
               const int rr = min(s + TK - 1, rows - 1);
-              v = A.kout[((size_t)pK * rows + rr) * (LANES + 1) + slot];
+              v = ld_dev4(&A.kout[((size_t)pK * rows + rr) * (LANES + 1) + slot]);
             }
             if (slot == 0) {
               float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
               if (pC >= 0) {
                 const int rr = min(s + 63 + TK, rows - 1);
-                c = A.kout[((size_t)pC * rows + rr) * (LANES + 1) + LANES];
+                const float* cp = reinterpret_cast<const float*>(&A.kout[((size_t)pC * rows + rr) * (LANES + 1) + LANES]);
+                c.y = ld_dev(cp + 1);
+                c.z = ld_dev(cp + 2);
               }
               v.y = c.y;
               v.z = c.z;
@@ -191,7 +211,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           const int s = blk * BLK + u;
           if (pG >= 0) {
             const int rr = min(s + 63, rows - 1);
-            v = A.gout[((size_t)pG * rows + rr) * TK + b];
+            v = ld_dev4(&A.gout[((size_t)pG * rows + rr) * TK + b]);
           }
         }
         stage[q] = v;
@@ -225,9 +245,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
       if (wave == TK - 1) {
         for (int e = lane; e < BLK * TK; e += LANES) {
           const int u = e / TK, b = e % TK;
-          A.gout[((size_t)me * rows + blk * BLK + u) * TK + b] = Gos[blk & 1][u][b];
+          const float4 v = Gos[blk & 1][u][b];
+          float* d = reinterpret_cast<float*>(&A.gout[((size_t)me * rows + blk * BLK + u) * TK + b]);
+          st_dev(d, v.x);
+          st_dev(d + 1, v.y);
+          st_dev(d + 2, v.z);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        wait_stores();
         if (lane == 0) __hip_atomic_store(&A.prog[me], (blk + 1) * BLK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     };
@@ -284,11 +308,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
         pn = pnew;
         // ---- the reference cell update (fluid.go:196-229)
         const uint32_t f = fcur[u];
-        if ((f & CF_GS) && j >= 0 && j <= NY - 1) {
-          const float sx0 = (f & CF_X0) ? 1.0f : 0.0f;
-          const float sx1 = (f & CF_X1) ? 1.0f : 0.0f;
-          const float sy0 = (f & CF_Y0) ? 1.0f : 0.0f;
-          const float sy1 = (f & CF_Y1) ? 1.0f : 0.0f;
+        if (f & CF_GS) {
+          // spread neighbour bits 1..4 to bytes 0..3, then convert each byte (0 or 1)
+          const uint32_t w = (((f >> 1) & 0xFu) * 0x00204081u) & 0x01010101u;
+          const float sx0 = (float)(w & 0xFFu);
+          const float sx1 = (float)((w >> 8) & 0xFFu);
+          const float sy0 = (float)((w >> 16) & 0xFFu);
+          const float sy1 = (float)(w >> 24);
           const float sum = sx0 + sx1 + sy0 + sy1;
           const float div = Ur - Ul + Vt - Vb;
           const float ad = fabsf(div);
@@ -315,19 +341,19 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           }
         }
         if (wave == TK - 1) {
-          float4* kr = A.kout + ((size_t)me * rows + s) * (LANES + 1);
-          kr[lane].x = Ul;
-          kr[lane + 1].y = Vb;
-          kr[lane + 1].z = p;
+          float* kr = reinterpret_cast<float*>(A.kout + ((size_t)me * rows + s) * (LANES + 1));
+          st_dev(kr + 4 * lane, Ul);
+          st_dev(kr + 4 * lane + 5, Vb);
+          st_dev(kr + 4 * lane + 6, p);
         }
         if (last && real_row && j >= 0 && j <= NY - 1) {
           const size_t c = (size_t)row * NY + j;
           const bool tu = row >= 1 && j >= 1 && j <= NY - 2;
           const bool tv = row >= 1 && row <= NX - 2 && j >= 1;
           const bool tp = row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2;
-          A.Uout[c] = tu ? Ul : A.Uin[c];
-          A.Vout[c] = tv ? Vb : A.Vin[c];
-          A.Pout[c] = tp ? p : A.Pin[c];
+          if (tu) A.Uout[c] = Ul;
+          if (tv) A.Vout[c] = Vb;
+          if (tp) A.Pout[c] = p;
         }
         __syncthreads();
       }
@@ -345,6 +371,34 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
     if (lane == 0 && wave_on && m > 0.0f) atomicMax(&A.maxdiv[k], __float_as_uint(m));
     __syncthreads();
   }
+}
+
+// Faces and cells no interior cell touches keep their input values: U with i = 0 or
+// j in {0, NY-1}; V with i in {0, NX-1} or j = 0; p on the ring. All lie on the ring.
+__global__ void k_gs_ring(int NX, int NY, const float* __restrict__ Uin, const float* __restrict__ Vin,
+                          const float* __restrict__ Pin, float* __restrict__ Uout, float* __restrict__ Vout,
+                          float* __restrict__ Pout) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nring = 2 * NX + 2 * (NY - 2);
+  if (t >= nring) return;
+  int i, j;
+  if (t < NX) {
+    i = t;
+    j = 0;
+  } else if (t < 2 * NX) {
+    i = t - NX;
+    j = NY - 1;
+  } else if (t < 2 * NX + NY - 2) {
+    i = 0;
+    j = t - 2 * NX + 1;
+  } else {
+    i = NX - 1;
+    j = t - 2 * NX - (NY - 2) + 1;
+  }
+  const size_t c = (size_t)i * NY + j;
+  if (!(i >= 1 && j >= 1 && j <= NY - 2)) Uout[c] = Uin[c];
+  if (!(i >= 1 && i <= NX - 2 && j >= 1)) Vout[c] = Vin[c];
+  Pout[c] = Pin[c];
 }
 
 // Zero the counters of both runs and the per-sweep maxima (graph-safe: no memset).
@@ -441,6 +495,9 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   if (rc) return rc;
   if (c->flags_dirty && (rc = launch_flags(c))) return rc;
   const int nprog = W.G * W.KB;
+  const int nring = 2 * c->NX + 2 * (c->NY - 2);
+  hipLaunchKernelGGL(k_gs_ring, dim3((nring + 255) / 256), dim3(256), 0, c->stream, c->NX, c->NY, c->U, c->V, c->p,
+                     c->U2, c->V2, c->T3);
   hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, nprog, W.d_ticket, c->d_maxdiv,
                      2 * c->nmax, c->d_ctrl);
   FL_HIP(hipGetLastError());

@@ -47,6 +47,18 @@ def build(kind, make, size, iters):
     raise SystemExit("unknown config %s" % kind)
 
 
+def measured_traffic(kind, size, iters):
+    """HBM bytes per GS launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE for this
+    workload (profiles/hbm_traffic_cavity4096.txt), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_%s%d.txt" % (kind, size))
+    if iters != 200 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("k_gs_wave traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(kind, size, iters, threads):
     """The float32 oracle stepping the same scene on this host's cores: a bounded sample,
     one step of the workload at the largest power-of-two size whose step is expected to
@@ -78,6 +90,8 @@ def main():
     ap.add_argument("--block-sweeps", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    ap.add_argument("--calibrate", action="store_true",
+                    help="first run a 256 MiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU decomposition "
@@ -86,6 +100,14 @@ def main():
 
     from fluid_amd import Fluid
     from fluid_amd import configs as C
+
+    if args.calibrate:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     print("bench: building %s %d^2, %d sweeps" % (args.config, args.size, args.iters), file=sys.stderr, flush=True)
     f = build(args.config, lambda *a: Fluid(*a), args.size, args.iters)
@@ -138,7 +160,7 @@ def main():
             args.config, W, H, sweeps, "on" if f.UseBFECC else "off"), "grid": [W, H], "pressure_sweeps": sweeps,
             "solver": ["auto", "hyperplane", "temporal-blocked"][args.solver]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, args.size, args.iters),
                      "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},
         "step_alg_bytes_per_cell": b_alg,
         "step_achieved_GBs": step_gbs,

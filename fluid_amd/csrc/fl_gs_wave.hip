@@ -54,8 +54,8 @@ struct GsArgs {
   const uint8_t* fdiag;          // flags by anti-diagonal: [(i+j)*NX + i]
   const float* omega;
   float damping, cp;
-  float4* kout;                  // [G*KB][rows][65]
-  float4* gout;                  // [G*KB][rows][TK]
+  float* kout;                   // [G*KB][rows][3][65]: planes U, V, p per row
+  float* gout;                   // [G*KB][rows][3][TK]
   int32_t* prog;                 // [G*KB]
   int32_t* ticket;
   uint32_t* maxdiv;              // [K]
@@ -77,9 +77,9 @@ __device__ __forceinline__ float ld_dev(const float* p) {
 __device__ __forceinline__ void st_dev(float* p, float v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ float4 ld_dev4(const float4* p) {
-  const float* f = reinterpret_cast<const float*>(p);
-  return make_float4(ld_dev(f), ld_dev(f + 1), ld_dev(f + 2), ld_dev(f + 3));
+// One handoff entry (U, V, p) stored as three planes of `stride` floats.
+__device__ __forceinline__ float4 ld_dev3(const float* p, int stride) {
+  return make_float4(ld_dev(p), ld_dev(p + stride), ld_dev(p + 2 * stride), 0.0f);
 }
 // Wait until every store this wave issued has been acknowledged by the memory system.
 __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -191,15 +191,15 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           } else {
             if (pK >= 0) {
               const int rr = min(s + TK - 1, rows - 1);
-              v = ld_dev4(&A.kout[((size_t)pK * rows + rr) * (LANES + 1) + slot]);
+              v = ld_dev3(A.kout + ((size_t)pK * rows + rr) * 3 * (LANES + 1) + slot, LANES + 1);
             }
             if (slot == 0) {
               float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
               if (pC >= 0) {
                 const int rr = min(s + 63 + TK, rows - 1);
-                const float* cp = reinterpret_cast<const float*>(&A.kout[((size_t)pC * rows + rr) * (LANES + 1) + LANES]);
-                c.y = ld_dev(cp + 1);
-                c.z = ld_dev(cp + 2);
+                const float* cp = A.kout + ((size_t)pC * rows + rr) * 3 * (LANES + 1) + LANES;
+                c.y = ld_dev(cp + (LANES + 1));
+                c.z = ld_dev(cp + 2 * (LANES + 1));
               }
               v.y = c.y;
               v.z = c.z;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           const int s = blk * BLK + u;
           if (pG >= 0) {
             const int rr = min(s + 63, rows - 1);
-            v = ld_dev4(&A.gout[((size_t)pG * rows + rr) * TK + b]);
+            v = ld_dev3(A.gout + ((size_t)pG * rows + rr) * 3 * TK + b, TK);
           }
         }
         stage[q] = v;
@@ -246,10 +246,10 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
         for (int e = lane; e < BLK * TK; e += LANES) {
           const int u = e / TK, b = e % TK;
           const float4 v = Gos[blk & 1][u][b];
-          float* d = reinterpret_cast<float*>(&A.gout[((size_t)me * rows + blk * BLK + u) * TK + b]);
+          float* d = A.gout + ((size_t)me * rows + blk * BLK + u) * 3 * TK + b;
           st_dev(d, v.x);
-          st_dev(d + 1, v.y);
-          st_dev(d + 2, v.z);
+          st_dev(d + TK, v.y);
+          st_dev(d + 2 * TK, v.z);
         }
         wait_stores();
         if (lane == 0) __hip_atomic_store(&A.prog[me], (blk + 1) * BLK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -341,10 +341,10 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           }
         }
         if (wave == TK - 1) {
-          float* kr = reinterpret_cast<float*>(A.kout + ((size_t)me * rows + s) * (LANES + 1));
-          st_dev(kr + 4 * lane, Ul);
-          st_dev(kr + 4 * lane + 5, Vb);
-          st_dev(kr + 4 * lane + 6, p);
+          float* kr = A.kout + ((size_t)me * rows + s) * 3 * (LANES + 1);
+          st_dev(kr + lane, Ul);
+          st_dev(kr + (LANES + 1) + lane + 1, Vb);
+          st_dev(kr + 2 * (LANES + 1) + lane + 1, p);
         }
         if (last && real_row && j >= 0 && j <= NY - 1) {
           const size_t c = (size_t)row * NY + j;
@@ -466,11 +466,11 @@ static int layout(Ctx* c, int N) {
   for (auto& o : order) tiles.push_back(o.second);
   W.ntiles = (int)tiles.size();
   const size_t nt = (size_t)W.G * W.KB;
-  W.kout_elems = nt * W.rows * (LANES + 1);
-  W.gout_elems = nt * W.rows * tk;
+  W.kout_elems = nt * W.rows * 3 * (LANES + 1);
+  W.gout_elems = nt * W.rows * 3 * tk;
   FL_HIP(hipMalloc(&W.d_tile, sizeof(int32_t) * std::max(1, W.ntiles)));
-  FL_HIP(hipMalloc(&W.d_kout, sizeof(float4) * W.kout_elems));
-  FL_HIP(hipMalloc(&W.d_gout, sizeof(float4) * W.gout_elems));
+  FL_HIP(hipMalloc(&W.d_kout, sizeof(float) * W.kout_elems));
+  FL_HIP(hipMalloc(&W.d_gout, sizeof(float) * W.gout_elems));
   FL_HIP(hipMalloc(&W.d_prog, sizeof(int32_t) * 2 * nt));
   FL_HIP(hipMalloc(&W.d_ticket, sizeof(int32_t) * 2));
   FL_HIP(hipMemcpy(W.d_tile, tiles.data(), sizeof(int32_t) * W.ntiles, hipMemcpyHostToDevice));

@@ -55,8 +55,8 @@ struct GsWork {
   int ntiles = 0;          // non-empty tiles in ticket order
   int grid = 0;            // persistent workgroups launched
   int32_t* d_tile = nullptr;    // [ntiles] packed (g << 16 | kb), ticket order
-  float4* d_kout = nullptr;     // [G*KB][rows][65]
-  float4* d_gout = nullptr;     // [G*KB][rows][tk]
+  float* d_kout = nullptr;      // [G*KB][rows][3][65]  K-stream handoff planes U, V, p
+  float* d_gout = nullptr;      // [G*KB][rows][3][tk]  G-stream handoff planes
   int32_t* d_prog = nullptr;    // [2][G*KB] progress counters (main run, rerun)
   int32_t* d_ticket = nullptr;  // [2]
   size_t kout_elems = 0, gout_elems = 0;

@@ -99,6 +99,17 @@ SIGNATURES = {
     "fl_scene_update": (_i, [_vp, _f]),
     "fl_scene_draw_line": (_i, [_vp, _i, _i, _i, _i, _i]),
     "fl_scene_smoke_brush": (_i, [_vp, _i, _i, _i, _f]),
+    "fl_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
+    "fl_batch_create": (_i, [_i, _ip, _ip, _f, _f, ctypes.POINTER(_vp)]),
+    "fl_batch_destroy": (_i, [_vp]),
+    "fl_batch_size": (_i, [_vp, _ip]),
+    "fl_batch_member": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "fl_batch_get_params": (_i, [_vp, ctypes.POINTER(FlParams)]),
+    "fl_batch_set_params": (_i, [_vp, ctypes.POINTER(FlParams)]),
+    "fl_batch_update": (_i, [_vp, _f]),
+    "fl_batch_sync": (_i, [_vp]),
+    "fl_batch_set_graph": (_i, [_vp, _i]),
+    "fl_batch_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
 }
 
 _lib = None

@@ -1,0 +1,443 @@
+// Batched ensembles: many independent simulations of uneven grid sizes stepped together.
+//
+// A batch carves every field of its members out of shared pools. Each member is a full
+// context (fl_ctx) over its slice, so the whole single-simulation API (edits, presets,
+// sources, views and its own Simulate) works per member. fl_batch_update advances all
+// members with one launch per pass: cell kernels take (member, 64x4 block) items from a
+// work-list, and the pressure solve is one persistent launch over the merged tile list of
+// every member (fl_gs_wave.hip). A batched step copies results back instead of swapping
+// buffers, so the member descriptors stay fixed and the step can be captured once as a
+// HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "fl_internal.h"
+
+namespace fl {
+
+static constexpr int MEMBER_NMAX = 1024;  // sweeps a member supports (size of its max|div| record)
+
+static int berr(int code, const std::string& m) {
+  set_error(m);
+  return code;
+}
+
+static void batch_free(Batch* B) {
+  if (!B) return;
+  (void)hipSetDevice(B->dev);
+  if (B->graph) (void)hipGraphExecDestroy(B->graph);
+  for (Ctx* c : B->members) free_ctx(c);
+  for (float* p : B->pool)
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)B->flags_pool, (void*)B->diag_pool, (void*)B->ctrl_pool, (void*)B->maxdiv_pool,
+                  (void*)B->d_sims, (void*)B->d_cells, (void*)B->d_rings, (void*)B->d_tiles, (void*)B->kout,
+                  (void*)B->gout, (void*)B->prog, (void*)B->ticket, (void*)B->d_omega, (void*)B->d_src,
+                  (void*)B->d_src_seg, (void*)B->d_src_sim})
+    if (p) (void)hipFree(p);
+  if (B->stream) (void)hipStreamDestroy(B->stream);
+  delete B;
+}
+
+// ---- per-step preparation (host side, outside any graph) -----------------------------------
+static int upload_omega(Batch* B, int N) {
+  if (B->omega_n == N && B->omega_relax == B->prm.relaxation) return FL_OK;
+  FL_HIP(hipStreamSynchronize(B->stream));  // earlier steps may still read the table
+  std::vector<float> om(std::max(N, 1));
+  const float init = B->prm.relaxation, minR = 1.2f;
+  for (int k = 0; k < N; k++) om[k] = init - (init - minR) * ((float)k / (float)N);
+  FL_HIP(hipMemcpy(B->d_omega, om.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+  B->omega_n = N;
+  B->omega_relax = B->prm.relaxation;
+  return FL_OK;
+}
+
+static int build_batch_sources(Batch* B) {
+  bool dirty = false;
+  for (size_t e = 0; e < B->members.size(); e++)
+    if (B->members[e]->src_gen != B->src_gen_seen[e]) dirty = true;
+  if (!dirty) return FL_OK;
+  FL_HIP(hipStreamSynchronize(B->stream));  // earlier steps may still read the program
+  std::vector<SrcOp> all;
+  std::vector<int32_t> seg, segsim;
+  for (size_t e = 0; e < B->members.size(); e++) {
+    std::vector<SrcOp> ops;
+    source_ops(B->members[e], ops);
+    std::stable_sort(ops.begin(), ops.end(), [](const SrcOp& a, const SrcOp& b) { return a.cell < b.cell; });
+    for (size_t k = 0; k < ops.size(); k++) {
+      if (k == 0 || ops[k].cell != ops[k - 1].cell) {
+        seg.push_back((int32_t)all.size());
+        segsim.push_back((int32_t)e);
+      }
+      all.push_back(ops[k]);
+    }
+    B->src_gen_seen[e] = B->members[e]->src_gen;
+  }
+  const int nseg = (int)seg.size();
+  seg.push_back((int32_t)all.size());
+  if (all.size() + 1 > B->src_cap || (size_t)nseg + 1 > B->src_cap) {
+    for (void* p : {(void*)B->d_src, (void*)B->d_src_seg, (void*)B->d_src_sim})
+      if (p) FL_HIP(hipFree(p));
+    const size_t cap = std::max<size_t>(all.size() + 1, 1024);
+    FL_HIP(hipMalloc(&B->d_src, sizeof(SrcOp) * cap));
+    FL_HIP(hipMalloc(&B->d_src_seg, sizeof(int32_t) * (cap + 1)));
+    FL_HIP(hipMalloc(&B->d_src_sim, sizeof(int32_t) * (cap + 1)));
+    B->src_cap = cap;
+  }
+  if (!all.empty()) {
+    FL_HIP(hipMemcpy(B->d_src, all.data(), sizeof(SrcOp) * all.size(), hipMemcpyHostToDevice));
+    FL_HIP(hipMemcpy(B->d_src_seg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice));
+    FL_HIP(hipMemcpy(B->d_src_sim, segsim.data(), sizeof(int32_t) * segsim.size(), hipMemcpyHostToDevice));
+  }
+  B->n_src_seg = nseg;
+  B->graph_gen++;
+  return FL_OK;
+}
+
+static int prepare_batch(Batch* B, float dt) {
+  int rc;
+  bool any_flags = false;
+  for (Ctx* c : B->members) {
+    if ((rc = flush_edits(c))) return rc;
+    if (c->flags_dirty) any_flags = true;
+  }
+  const int N = B->prm.pressure_iters;
+  if (N > MEMBER_NMAX) return berr(FL_ERR_UNSUPPORTED, "batched solves support at most 1024 sweeps");
+  if (B->prm.use_multigrid && B->prm.multigrid_levels > 1)
+    return berr(FL_ERR_UNSUPPORTED, "UseMultigrid: the multigrid V-cycle is not implemented on the device");
+  if ((rc = upload_omega(B, N))) return rc;
+  // descriptors of all members with this step's cp and the GS layout for N sweeps
+  std::vector<SimDesc> d(B->members.size());
+  for (size_t e = 0; e < B->members.size(); e++) {
+    Ctx* c = B->members[e];
+    d[e] = c->desc();
+    d[e].cp = c->density * c->h / dt;
+    d[e].early_exit = B->prm.early_exit;
+    d[e].nmax = MEMBER_NMAX;
+  }
+  // the batch's own tile layout replaces whatever GS workspace a member holds from
+  // stepping on its own
+  const int old_n = B->gs_n, old_tk = B->tk;
+  if (N > 0 && (rc = batch_gs_layout(B, N, d))) return rc;
+  if (N > 0 && (B->gs_n != old_n || B->tk != old_tk)) B->graph_gen++;
+  if (B->descs.size() != d.size() || std::memcmp(B->descs.data(), d.data(), sizeof(SimDesc) * d.size()) != 0) {
+    FL_HIP(hipStreamSynchronize(B->stream));  // earlier steps may still read the descriptors
+    FL_HIP(hipMemcpy(B->d_sims, d.data(), sizeof(SimDesc) * d.size(), hipMemcpyHostToDevice));
+    B->descs = d;
+  }
+  if (any_flags) {
+    if ((rc = batch_flags(B))) return rc;
+    for (Ctx* c : B->members) c->flags_dirty = false;
+  }
+  return build_batch_sources(B);
+}
+
+// Simulate (fluid.go:79-109) of every member, preceded by each member's per-step sources
+// and jet (main.go:231-241); the results end in the members' own buffers.
+static int enqueue_step(Batch* B, float dt) {
+  int rc;
+  const fl_params& P = B->prm;
+  if ((rc = batch_apply_sources(B))) return rc;
+  if ((rc = batch_fill_role(B, B_P, 0.0f))) return rc;
+  if (P.viscosity_diffusion > 0) {
+    if ((rc = batch_viscosity(B, dt))) return rc;
+    if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
+  }
+  if (P.pressure_iters > 0) {
+    if ((rc = batch_gs_solve(B, P.pressure_iters))) return rc;
+    if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2)) ||
+        (rc = batch_copy_role(B, B_P, B_T3)))
+      return rc;
+  }
+  if ((rc = batch_conf_turb(B, dt))) return rc;
+  if ((rc = batch_borders(B))) return rc;
+  if (P.use_bfecc) {
+    if ((rc = batch_advect_velocity(B, B_U, B_V, B_U2, B_V2, dt))) return rc;
+    if ((rc = batch_bfecc_velocity_corr(B, dt))) return rc;
+    if ((rc = batch_advect_velocity(B, B_T0, B_T1, B_U, B_V, dt))) return rc;
+    if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
+    if ((rc = batch_bfecc_smoke_corr(B, dt))) return rc;
+    return batch_advect_smoke(B, B_T2, B_M, dt);
+  }
+  if ((rc = batch_advect_velocity(B, B_U, B_V, B_U2, B_V2, dt))) return rc;
+  if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
+  if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
+  return batch_copy_role(B, B_M, B_M2);
+}
+
+static int batch_update(Batch* B, float dt) {
+  int rc = prepare_batch(B, dt);
+  if (rc) return rc;
+  if (B->use_graph) {
+    const bool same = B->graph && B->graph_dt == dt && B->graph_key_gen == B->graph_gen &&
+                      std::memcmp(&B->graph_prm, &B->prm, sizeof(fl_params)) == 0;
+    if (!same) {
+      if (B->graph) {
+        FL_HIP(hipGraphExecDestroy(B->graph));
+        B->graph = nullptr;
+      }
+      hipGraph_t g;
+      FL_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_step(B, dt);
+      hipError_t e = hipStreamEndCapture(B->stream, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return berr(FL_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return berr(FL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      B->graph_dt = dt;
+      B->graph_key_gen = B->graph_gen;
+      B->graph_prm = B->prm;
+    }
+    FL_HIP(hipGraphLaunch(B->graph, B->stream));
+  } else {
+    if ((rc = enqueue_step(B, dt))) return rc;
+  }
+  for (Ctx* c : B->members) {
+    c->last_n = B->prm.pressure_iters;
+    c->host_sweeps = -1;
+  }
+  B->steps++;
+  return FL_OK;
+}
+
+// ---- autotuning of the Gauss-Seidel tile depth ---------------------------------------------
+// The solve reads U, V, p and writes only scratch outputs (U2, V2, T3) and its statistics,
+// so candidates are timed on the live state without changing it.
+static float time_ms(hipStream_t s, const std::function<int()>& fn, int reps, int* rc) {
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  *rc = fn();  // warm-up (layout, first touch)
+  (void)hipEventRecord(a, s);
+  for (int r = 0; r < reps && !*rc; r++) *rc = fn();
+  (void)hipEventRecord(b, s);
+  (void)hipEventSynchronize(b);
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+#define B_OF(x) reinterpret_cast<Batch*>(x)
+
+extern "C" {
+
+int fl_batch_create(int n, const int* widths, const int* heights, float density, float h, fl_batch** out) {
+  if (!out || n < 1 || !widths || !heights) return berr(FL_ERR_INVALID_ARG, "bad batch arguments");
+  *out = nullptr;
+  if (!(h > 0.0f)) return berr(FL_ERR_INVALID_ARG, "h must be > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return berr(FL_ERR_NO_DEVICE, "no HIP device available: libfluidhip requires an MI355X (gfx950) GPU");
+  Batch* B = new Batch();
+  if (hipGetDevice(&B->dev) != hipSuccess) B->dev = 0;
+  B->prm = default_params();
+  B->nmax = MEMBER_NMAX;
+  auto fail = [&](int code, const std::string& m) {
+    batch_free(B);
+    return berr(code, m);
+  };
+  size_t ncells = 0, ndiag = 0;
+  for (int e = 0; e < n; e++) {
+    if (widths[e] < 1 || heights[e] < 1) return fail(FL_ERR_INVALID_ARG, "member sizes must be >= 1");
+    ncells += (size_t)(widths[e] + 2) * (heights[e] + 2);
+    ndiag += (size_t)(widths[e] + heights[e] + 3) * (widths[e] + 2);
+  }
+  if (ncells * NBUF > ((size_t)1 << 31) * 8) return fail(FL_ERR_INVALID_ARG, "batch too large");
+  B->ncells = ncells;
+  hipError_t er = hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking);
+  if (er != hipSuccess) return fail(FL_ERR_HIP, "hipStreamCreate");
+  for (int k = 0; k < NBUF; k++) {
+    if (hipMalloc(&B->pool[k], sizeof(float) * ncells) != hipSuccess) return fail(FL_ERR_HIP, "hipMalloc pool");
+    (void)hipMemsetAsync(B->pool[k], 0, sizeof(float) * ncells, B->stream);
+  }
+  if (hipMalloc(&B->flags_pool, ncells) != hipSuccess || hipMalloc(&B->diag_pool, ndiag) != hipSuccess ||
+      hipMalloc(&B->ctrl_pool, sizeof(int32_t) * 4 * n) != hipSuccess ||
+      hipMalloc(&B->maxdiv_pool, sizeof(uint32_t) * 2 * MEMBER_NMAX * n) != hipSuccess ||
+      hipMalloc(&B->d_sims, sizeof(SimDesc) * n) != hipSuccess ||
+      hipMalloc(&B->d_omega, sizeof(float) * MEMBER_NMAX) != hipSuccess)
+    return fail(FL_ERR_HIP, "hipMalloc batch");
+  (void)hipMemsetAsync(B->diag_pool, 0, ndiag, B->stream);
+  (void)hipMemsetAsync(B->ctrl_pool, 0, sizeof(int32_t) * 4 * n, B->stream);
+  (void)hipMemsetAsync(B->maxdiv_pool, 0, sizeof(uint32_t) * 2 * MEMBER_NMAX * n, B->stream);
+  std::vector<CellItem> cells;
+  std::vector<RingItem> rings;
+  size_t off = 0, doff = 0;
+  for (int e = 0; e < n; e++) {
+    Ctx* c = new Ctx();
+    c->batch = B;
+    c->member = e;
+    c->dev = B->dev;
+    c->stream = B->stream;
+    c->NX = widths[e] + 2;
+    c->NY = heights[e] + 2;
+    c->N = (size_t)c->NX * c->NY;
+    c->density = density;
+    c->h = h;
+    c->prm = default_params();
+    c->grid = Grid{c->NX, c->NY, h, 1.0f / h, 0.5f * h, h / 2.0f, (float)c->NX * h, (float)c->NY * h};
+    c->hS.assign(c->N, 0.0f);
+    float** fields[NBUF] = {&c->U, &c->V, &c->nU, &c->nV, &c->p, &c->S, &c->M, &c->nM,
+                            &c->U2, &c->V2, &c->M2, &c->T0, &c->T1, &c->T2, &c->T3};
+    for (int k = 0; k < NBUF; k++) *fields[k] = B->pool[k] + off;
+    c->flags = B->flags_pool + off;
+    c->flags_diag = B->diag_pool + doff;
+    c->d_ctrl = B->ctrl_pool + 4 * e;
+    c->d_maxdiv = B->maxdiv_pool + (size_t)2 * MEMBER_NMAX * e;
+    c->nmax = MEMBER_NMAX;
+    B->members.push_back(c);
+    if (hipMalloc(&c->d_omega, sizeof(float) * MEMBER_NMAX) != hipSuccess ||
+        hipMalloc(&c->d_red, 64) != hipSuccess)
+      return fail(FL_ERR_HIP, "hipMalloc member");
+    for (int by = 0; by < (c->NX + 3) / 4; by++)
+      for (int bx = 0; bx < (c->NY + 63) / 64; bx++) cells.push_back(CellItem{e, bx, by, 0});
+    const int m = std::max(c->NX, c->NY);
+    for (int blk = 0; blk < (m + 255) / 256; blk++) rings.push_back(RingItem{e, blk});
+    off += c->N;
+    doff += (size_t)(c->NX + c->NY - 1) * c->NX;
+  }
+  B->src_gen_seen.assign(n, -1);
+  B->ncell_items = (int)cells.size();
+  B->nring_items = (int)rings.size();
+  if (hipMalloc(&B->d_cells, sizeof(CellItem) * cells.size()) != hipSuccess ||
+      hipMalloc(&B->d_rings, sizeof(RingItem) * rings.size()) != hipSuccess)
+    return fail(FL_ERR_HIP, "hipMalloc work-lists");
+  (void)hipMemcpy(B->d_cells, cells.data(), sizeof(CellItem) * cells.size(), hipMemcpyHostToDevice);
+  (void)hipMemcpy(B->d_rings, rings.data(), sizeof(RingItem) * rings.size(), hipMemcpyHostToDevice);
+  if (hipStreamSynchronize(B->stream) != hipSuccess) return fail(FL_ERR_HIP, "batch init");
+  *out = reinterpret_cast<fl_batch*>(B);
+  return FL_OK;
+}
+
+int fl_batch_destroy(fl_batch* b) {
+  batch_free(B_OF(b));
+  return FL_OK;
+}
+
+int fl_batch_size(fl_batch* b, int* n) {
+  if (!b || !n) return berr(FL_ERR_INVALID_ARG, "null argument");
+  *n = (int)B_OF(b)->members.size();
+  return FL_OK;
+}
+
+int fl_batch_member(fl_batch* b, int index, fl_ctx** out) {
+  if (!b || !out) return berr(FL_ERR_INVALID_ARG, "null argument");
+  Batch* B = B_OF(b);
+  if (index < 0 || index >= (int)B->members.size()) return berr(FL_ERR_INVALID_ARG, "member index out of range");
+  *out = reinterpret_cast<fl_ctx*>(B->members[index]);
+  return FL_OK;
+}
+
+int fl_batch_get_params(fl_batch* b, fl_params* p) {
+  if (!b || !p) return berr(FL_ERR_INVALID_ARG, "null argument");
+  *p = B_OF(b)->prm;
+  return FL_OK;
+}
+
+int fl_batch_set_params(fl_batch* b, const fl_params* p) {
+  if (!b || !p) return berr(FL_ERR_INVALID_ARG, "null argument");
+  if (p->pressure_iters < 0 || p->pressure_iters > MEMBER_NMAX)
+    return berr(FL_ERR_INVALID_ARG, "pressure_iters must be in [0, 1024] for a batch");
+  B_OF(b)->prm = *p;
+  return FL_OK;
+}
+
+int fl_batch_update(fl_batch* b, float dt) {
+  if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
+  Batch* B = B_OF(b);
+  if (hipSetDevice(B->dev) != hipSuccess) return berr(FL_ERR_HIP, "hipSetDevice");
+  for (Ctx* c : B->members)
+    if (c->NX < 3 || c->NY < 3) return berr(FL_ERR_UNSUPPORTED, "members need at least 1x1 interior cells");
+  return batch_update(B, dt);
+}
+
+int fl_batch_sync(fl_batch* b) {
+  if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
+  FL_HIP(hipStreamSynchronize(B_OF(b)->stream));
+  return FL_OK;
+}
+
+int fl_batch_set_graph(fl_batch* b, int on) {
+  if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
+  Batch* B = B_OF(b);
+  B->use_graph = on != 0;
+  if (!B->use_graph && B->graph) {
+    FL_HIP(hipGraphExecDestroy(B->graph));
+    B->graph = nullptr;
+  }
+  return FL_OK;
+}
+
+int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
+  if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
+  Batch* B = B_OF(b);
+  if (B->prm.pressure_iters <= 0) return FL_OK;
+  int best = 0, rc = 0;
+  float best_ms = 1e30f;
+  const fl_params saved = B->prm;
+  for (int tk : {8, 16}) {
+    B->prm.block_sweeps = tk;
+    if ((rc = prepare_batch(B, dt))) return rc;
+    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters); }, 2, &rc);
+    if (rc) return rc;
+    if (t < best_ms) {
+      best_ms = t;
+      best = tk;
+    }
+  }
+  B->prm = saved;
+  B->prm.block_sweeps = best;
+  if ((rc = prepare_batch(B, dt))) return rc;
+  if (block_sweeps) *block_sweeps = best;
+  if (ms) *ms = best_ms;
+  return FL_OK;
+}
+
+int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms) {
+  if (!ctx) return berr(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const int N = c->prm.pressure_iters;
+  if (N <= 0 || c->prm.solver == 1 || (c->prm.use_multigrid && c->prm.multigrid_levels > 1)) return FL_OK;
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  if (c->flags_dirty && (rc = launch_flags(c))) return rc;
+  int best = 0;
+  float best_ms = 1e30f;
+  const int saved_solver = c->prm.solver;
+  c->prm.solver = 2;
+  for (int tk : {8, 16}) {
+    c->prm.block_sweeps = tk;
+    // the solve swaps U/U2, V/V2, p/T3 towards its outputs; swapping back restores the state
+    float t = time_ms(c->stream,
+                      [&]() {
+                        int r = pressure_solve(c, N, dt);
+                        std::swap(c->U, c->U2);
+                        std::swap(c->V, c->V2);
+                        std::swap(c->p, c->T3);
+                        return r;
+                      },
+                      2, &rc);
+    if (rc) return rc;
+    if (t < best_ms) {
+      best_ms = t;
+      best = tk;
+    }
+  }
+  c->prm.solver = saved_solver;
+  c->prm.block_sweeps = best;
+  if (block_sweeps) *block_sweeps = best;
+  if (ms) *ms = best_ms;
+  return FL_OK;
+}
+
+}  // extern "C"

@@ -59,7 +59,7 @@ static void push_op(Ctx* c, int field, size_t idx, bool add, float v) {
 }
 
 // Apply queued edits on the device: ops are grouped per target word, order kept.
-static int flush_edits(Ctx* c) {
+int flush_edits(Ctx* c) {
   if (c->edits.empty()) return FL_OK;
   const size_t n = c->edits.size();
   std::vector<uint32_t> order(n);
@@ -110,10 +110,9 @@ static int prepare(Ctx* c) {
   return FL_OK;
 }
 
-// Per-step source program: sources, sinks, then the jet (main.go:231-241,474-486).
-static int build_sources(Ctx* c) {
-  if (!c->src_dirty) return FL_OK;
-  std::vector<SrcOp> ops;
+// Per-step source ops of one context in the order the reference applies them: sources,
+// sinks, then the jet (main.go:231-241,474-486). Cells are local to the context.
+void source_ops(Ctx* c, std::vector<SrcOp>& ops) {
   auto inb = [&](int i, int j) { return i >= 0 && i < c->NX && j >= 0 && j < c->NY; };
   for (auto& s : c->sources) {
     if (!inb(s.I, s.J)) continue;
@@ -132,6 +131,12 @@ static int build_sources(Ctx* c) {
       ops.push_back(SrcOp{cell, 1, 0, c->jetSmoke, 0.0f});
     }
   }
+}
+
+static int build_sources(Ctx* c) {
+  if (!c->src_dirty) return FL_OK;
+  std::vector<SrcOp> ops;
+  source_ops(c, ops);
   std::stable_sort(ops.begin(), ops.end(), [](const SrcOp& a, const SrcOp& b) { return a.cell < b.cell; });
   std::vector<int32_t> seg;
   for (size_t k = 0; k < ops.size(); k++)
@@ -249,17 +254,21 @@ static int simulate(Ctx* c, float dt, bool start_marked = false) {
   return FL_OK;
 }
 
-static void free_ctx(Ctx* c) {
+void free_ctx(Ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
-  float** bufs[] = {&c->U, &c->V, &c->nU, &c->nV, &c->p, &c->S, &c->M, &c->nM, &c->U2, &c->V2,
-                    &c->M2, &c->T0, &c->T1, &c->T2, &c->T3, &c->d_omega, &c->d_edit_vals, &c->d_xy};
-  for (auto** b : bufs)
+  if (!c->batch) {  // a batch member's fields, flags, control block and stream belong to the batch
+    float** bufs[] = {&c->U, &c->V, &c->nU, &c->nV, &c->p, &c->S, &c->M, &c->nM,
+                      &c->U2, &c->V2, &c->M2, &c->T0, &c->T1, &c->T2, &c->T3};
+    for (auto** b : bufs)
+      if (*b) (void)hipFree(*b);
+    if (c->flags) (void)hipFree(c->flags);
+    if (c->flags_diag) (void)hipFree(c->flags_diag);
+    if (c->d_ctrl) (void)hipFree(c->d_ctrl);
+    if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
+  }
+  for (float** b : {&c->d_omega, &c->d_edit_vals, &c->d_xy})
     if (*b) (void)hipFree(*b);
-  if (c->flags) (void)hipFree(c->flags);
-  if (c->flags_diag) (void)hipFree(c->flags_diag);
-  if (c->d_ctrl) (void)hipFree(c->d_ctrl);
-  if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
   if (c->d_red) (void)hipFree(c->d_red);
   if (c->d_edit_keys) (void)hipFree(c->d_edit_keys);
   if (c->d_edit_seg) (void)hipFree(c->d_edit_seg);
@@ -268,11 +277,11 @@ static void free_ctx(Ctx* c) {
   pressure_free(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && !c->batch) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
-static fl_params default_params() {
+fl_params default_params() {
   fl_params p{};
   p.confinement = 0.0f;
   p.viscosity_diffusion = 0.0f;
@@ -415,6 +424,9 @@ int fl_create(float density, int width, int height, float h, fl_ctx** out) {
   if ((e = hipMalloc(&c->flags, c->N)) != hipSuccess) return fail(e, "hipMalloc flags");
   if ((e = hipMalloc(&c->flags_diag, (size_t)(c->NX + c->NY - 1) * c->NX)) != hipSuccess)
     return fail(e, "hipMalloc flags_diag");
+  // entries off the grid stay zero for the context's lifetime
+  if ((e = hipMemsetAsync(c->flags_diag, 0, (size_t)(c->NX + c->NY - 1) * c->NX, c->stream)) != hipSuccess)
+    return fail(e, "hipMemset flags_diag");
   if ((e = hipMalloc(&c->d_ctrl, 64)) != hipSuccess) return fail(e, "hipMalloc ctrl");
   if ((e = hipMemsetAsync(c->d_ctrl, 0, 64, c->stream)) != hipSuccess) return fail(e, "hipMemset ctrl");
   if ((e = hipMalloc(&c->d_red, 64)) != hipSuccess) return fail(e, "hipMalloc red");
@@ -426,6 +438,7 @@ int fl_create(float density, int width, int height, float h, fl_ctx** out) {
 
 int fl_destroy(fl_ctx* ctx) {
   if (!ctx) return FL_OK;
+  if (C_OF(ctx)->batch) return err(FL_ERR_INVALID_ARG, "batch members are destroyed with their batch");
   free_ctx(C_OF(ctx));
   return FL_OK;
 }
@@ -806,6 +819,7 @@ int fl_scene_preset(fl_ctx* ctx, int preset, float lid_u) {
   c->sources.clear();
   c->sinks.clear();
   c->src_dirty = true;
+  c->src_gen++;
   for (int i = 0; i < c->NX; i++)
     for (int j = 0; j < c->NY; j++) set_solid(c, i, j, false);
   if (preset == 0) {
@@ -838,6 +852,7 @@ int fl_scene_set_jet(fl_ctx* ctx, int on, int x, int j_lo, int j_hi, float u, fl
   c->jetU = u;
   c->jetSmoke = smoke;
   c->src_dirty = true;
+  c->src_gen++;
   return FL_OK;
 }
 int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v) {
@@ -845,6 +860,7 @@ int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v) {
   Ctx* c = C_OF(ctx);
   (sink ? c->sinks : c->sources).push_back(Source{i, j, u, v});
   c->src_dirty = true;
+  c->src_gen++;
   return FL_OK;
 }
 int fl_scene_clear_sources(fl_ctx* ctx) {
@@ -853,6 +869,7 @@ int fl_scene_clear_sources(fl_ctx* ctx) {
   c->sources.clear();
   c->sinks.clear();
   c->src_dirty = true;
+  c->src_gen++;
   return FL_OK;
 }
 int fl_scene_apply_sources(fl_ctx* ctx) {

@@ -41,25 +41,15 @@ static constexpr int LANES = 64;
 static constexpr int BLK = 8;                  // columns per staging/publish block
 static constexpr int SPIN_LIMIT = 1 << 22;     // bounded wait (about a second); then error and continue
 
-struct GsArgs {
-  int NX, NY, G, KB, rows, ntiles, which;
-  int K;                         // sweeps of the main run; the re-run reads ctrl[1]
-  const int32_t* tiles;          // ticket order, packed (g << 16) | kb
-  const float* Uin;
-  const float* Vin;
-  const float* Pin;              // initial p (accumulated onto, like the serial loop)
-  float* Uout;
-  float* Vout;
-  float* Pout;
-  const uint8_t* fdiag;          // flags by anti-diagonal: [(i+j)*NX + i]
-  const float* omega;
-  float damping, cp;
-  float* kout;                   // [G*KB][rows][3][65]: planes U, V, p per row
-  float* gout;                   // [G*KB][rows][3][TK]
-  int32_t* prog;                 // [G*KB]
+// One launch of the solve (main run or early-exit re-run).
+struct GsRun {
+  const int2* tiles;  // ticket order: (simulation, (g << 16) | kb)
+  int ntiles;
+  int which;          // 0 main run (K sweeps), 1 re-run (ctrl[1] sweeps of each simulation)
+  int K;
   int32_t* ticket;
-  uint32_t* maxdiv;              // [K]
-  int32_t* ctrl;
+  const float* omega;
+  float damping;
 };
 
 __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX) {
@@ -97,8 +87,11 @@ __device__ __forceinline__ int wait_progress(int32_t* p, int need, int32_t* err)
   return v;
 }
 
-template <int TK>
-__global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
+__device__ __forceinline__ const SimDesc& sim_desc(const OneSim& m, int) { return m.d; }
+__device__ __forceinline__ const SimDesc& sim_desc(const ManySims& m, int e) { return m.sims[e]; }
+
+template <int TK, class M>
+__global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
   const int lane = threadIdx.x & (LANES - 1);
   const int wave = threadIdx.x / LANES;
   const int tid = threadIdx.x;
@@ -112,24 +105,36 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
   __shared__ int s_tile;
   __shared__ int s_seen[3];
 
-  const int K = A.which ? A.ctrl[1] : A.K;
-  if (K <= 0) return;
-  const int NX = A.NX, NY = A.NY, rows = A.rows, KB = A.KB;
-  const int nblk = rows / BLK;
-  int32_t* err = A.ctrl + 3;
-
   for (;;) {
-    if (tid == 0) s_tile = atomicAdd(A.ticket, 1);
+    if (tid == 0) s_tile = atomicAdd(R.ticket, 1);
     __syncthreads();
-    const int t = s_tile;
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
     __syncthreads();
-    if (t >= A.ntiles) break;
-    const int g = A.tiles[t] >> 16, kb = A.tiles[t] & 0xffff;
+    if (t >= R.ntiles) break;
+    const int2 te = R.tiles[t];
+    const int sim = te.x, g = te.y >> 16, kb = te.y & 0xffff;
+    const SimDesc& D = sim_desc(m, sim);
+    const int NX = D.g.NX, NY = D.g.NY, rows = D.rows, KB = D.KB;
+    const int nblk = rows / BLK;
     const int me = g * KB + kb;
+    int32_t* prog = D.prog + R.which * D.G * KB;
+    int32_t* err = D.ctrl + 3;
+    const int K = R.which ? D.ctrl[1] : R.K;
     if (tile_empty(g, kb, K, TK, NX)) {
-      if (tid == 0) __hip_atomic_store(&A.prog[me], rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (tid == 0) __hip_atomic_store(&prog[me], rows, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       continue;
     }
+    const float* Uin = D.buf[B_U];
+    const float* Vin = D.buf[B_V];
+    const float* Pin = D.buf[B_P];
+    float* Uout = D.buf[B_U2];
+    float* Vout = D.buf[B_V2];
+    float* Pout = D.buf[B_T3];
+    const uint8_t* fdiag = D.diag;
+    float* kout = D.kout;
+    float* gout = D.gout;
+    const float cp = D.cp;
+    const float damping = R.damping;
     const int k0 = kb * TK;
     const int k = k0 + wave;
     const bool wave_on = k < K;
@@ -137,7 +142,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
     const int i0 = LANES * g - k0;           // row of (lane 0, wave 0)
     const int row = i0 + lane - wave;        // this thread's row
     const bool real_row = row >= 0 && row <= NX - 1;
-    const float om = wave_on ? A.omega[k] : 0.0f;
+    const float om = wave_on ? R.omega[k] : 0.0f;
     const int d0 = LANES * g - k0 - 2 * wave;  // anti-diagonal of column 0 for this wave
 
     // Producers: G stream (g-1,kb), K stream (g,kb-1), corner (g-1,kb-1).
@@ -153,15 +158,15 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
         const int s1 = blk * BLK + BLK - 1;  // last step of the block
         if (pG >= 0) {
           int need = min(s1 + 63 + 1, rows);
-          if (s_seen[0] < need) s_seen[0] = wait_progress(&A.prog[pG], need, err);
+          if (s_seen[0] < need) s_seen[0] = wait_progress(&prog[pG], need, err);
         }
         if (pK >= 0) {
           int need = min(s1 + TK - 1 + 1, rows);
-          if (s_seen[1] < need) s_seen[1] = wait_progress(&A.prog[pK], need, err);
+          if (s_seen[1] < need) s_seen[1] = wait_progress(&prog[pK], need, err);
         }
         if (pC >= 0) {
           int need = min(s1 + 63 + TK + 1, rows);
-          if (s_seen[2] < need) s_seen[2] = wait_progress(&A.prog[pC], need, err);
+          if (s_seen[2] < need) s_seen[2] = wait_progress(&prog[pC], need, err);
         }
       }
     };
@@ -182,24 +187,24 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
             // initial state: lane `slot` of wave 0 is row i0+slot at column s-slot
             const int r = i0 + slot, j = s - slot;
             if (slot < LANES && j >= 0 && j <= NY - 1) {
-              if (r + 1 >= 0 && r + 1 <= NX - 1) v.x = A.Uin[(size_t)(r + 1) * NY + j];
+              if (r + 1 >= 0 && r + 1 <= NX - 1) v.x = Uin[(size_t)(r + 1) * NY + j];
               if (r >= 0 && r <= NX - 1 && j + 1 <= NY - 1) {
-                v.y = A.Vin[(size_t)r * NY + j + 1];
-                v.z = A.Pin[(size_t)r * NY + j + 1];
+                v.y = Vin[(size_t)r * NY + j + 1];
+                v.z = Pin[(size_t)r * NY + j + 1];
               }
             }
           } else {
             if (pK >= 0) {
               const int rr = min(s + TK - 1, rows - 1);
-              v = ld_dev3(A.kout + ((size_t)pK * rows + rr) * 3 * (LANES + 1) + slot, LANES + 1);
+              v = ld_dev3(kout + ((size_t)pK * rows + rr) * 3 * (LANES + 1) + slot, LANES + 1);
             }
             if (slot == 0) {
               float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
               if (pC >= 0) {
                 const int rr = min(s + 63 + TK, rows - 1);
-                const float* cp = A.kout + ((size_t)pC * rows + rr) * 3 * (LANES + 1) + LANES;
-                c.y = ld_dev(cp + (LANES + 1));
-                c.z = ld_dev(cp + 2 * (LANES + 1));
+                const float* cpp = kout + ((size_t)pC * rows + rr) * 3 * (LANES + 1) + LANES;
+                c.y = ld_dev(cpp + (LANES + 1));
+                c.z = ld_dev(cpp + 2 * (LANES + 1));
               }
               v.y = c.y;
               v.z = c.z;
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           const int s = blk * BLK + u;
           if (pG >= 0) {
             const int rr = min(s + 63, rows - 1);
-            v = ld_dev3(A.gout + ((size_t)pG * rows + rr) * 3 * TK + b, TK);
+            v = ld_dev3(gout + ((size_t)pG * rows + rr) * 3 * TK + b, TK);
           }
         }
         stage[q] = v;
@@ -234,9 +239,9 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
 #pragma unroll
       for (int u = 0; u < BLK; u++) {
         const int s = blk * BLK + u;
-        const int d = d0 + s;
+        const int dd = d0 + s;
         uint32_t f = 0;
-        if (wave_on && real_row && d >= 0 && d <= NX + NY - 2) f = A.fdiag[(size_t)d * NX + row];
+        if (wave_on && real_row && dd >= 0 && dd <= NX + NY - 2) f = fdiag[(size_t)dd * NX + row];
         fl[u] = f;
       }
     };
@@ -246,13 +251,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
         for (int e = lane; e < BLK * TK; e += LANES) {
           const int u = e / TK, b = e % TK;
           const float4 v = Gos[blk & 1][u][b];
-          float* d = A.gout + ((size_t)me * rows + blk * BLK + u) * 3 * TK + b;
-          st_dev(d, v.x);
-          st_dev(d + TK, v.y);
-          st_dev(d + 2 * TK, v.z);
+          float* dst = gout + ((size_t)me * rows + blk * BLK + u) * 3 * TK + b;
+          st_dev(dst, v.x);
+          st_dev(dst + TK, v.y);
+          st_dev(dst + 2 * TK, v.z);
         }
         wait_stores();
-        if (lane == 0) __hip_atomic_store(&A.prog[me], (blk + 1) * BLK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_store(&prog[me], (blk + 1) * BLK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     };
 
@@ -321,8 +326,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           if (ad > md) md = ad;
           float q = -div / sum;
           q *= om;
-          q *= A.damping;
-          p += A.cp * q;
+          q *= damping;
+          p += cp * q;
           Ul -= sx0 * q;
           Ur += sx1 * q;
           Vb -= sy0 * q;
@@ -341,7 +346,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           }
         }
         if (wave == TK - 1) {
-          float* kr = A.kout + ((size_t)me * rows + s) * 3 * (LANES + 1);
+          float* kr = kout + ((size_t)me * rows + s) * 3 * (LANES + 1);
           st_dev(kr + lane, Ul);
           st_dev(kr + (LANES + 1) + lane + 1, Vb);
           st_dev(kr + 2 * (LANES + 1) + lane + 1, p);
@@ -351,9 +356,9 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
           const bool tu = row >= 1 && j >= 1 && j <= NY - 2;
           const bool tv = row >= 1 && row <= NX - 2 && j >= 1;
           const bool tp = row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2;
-          if (tu) A.Uout[c] = Ul;
-          if (tv) A.Vout[c] = Vb;
-          if (tp) A.Pout[c] = p;
+          if (tu) Uout[c] = Ul;
+          if (tv) Vout[c] = Vb;
+          if (tp) Pout[c] = p;
         }
         __syncthreads();
       }
@@ -366,71 +371,75 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(GsArgs A) {
     }
     publish(nblk - 1);
     // per-sweep max |div|
-    float m = md;
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if (lane == 0 && wave_on && m > 0.0f) atomicMax(&A.maxdiv[k], __float_as_uint(m));
+    float mv = md;
+    for (int o = 32; o > 0; o >>= 1) mv = fmaxf(mv, __shfl_xor(mv, o, 64));
+    if (lane == 0 && wave_on && mv > 0.0f) atomicMax(&D.maxdiv[R.which * D.nmax + k], __float_as_uint(mv));
     __syncthreads();
   }
 }
 
 // Faces and cells no interior cell touches keep their input values: U with i = 0 or
 // j in {0, NY-1}; V with i in {0, NX-1} or j = 0; p on the ring. All lie on the ring.
-__global__ void k_gs_ring(int NX, int NY, const float* __restrict__ Uin, const float* __restrict__ Vin,
-                          const float* __restrict__ Pin, float* __restrict__ Uout, float* __restrict__ Vout,
-                          float* __restrict__ Pout) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nring = 2 * NX + 2 * (NY - 2);
-  if (t >= nring) return;
-  int i, j;
-  if (t < NX) {
-    i = t;
-    j = 0;
-  } else if (t < 2 * NX) {
-    i = t - NX;
-    j = NY - 1;
-  } else if (t < 2 * NX + NY - 2) {
-    i = 0;
-    j = t - 2 * NX + 1;
-  } else {
-    i = NX - 1;
-    j = t - 2 * NX - (NY - 2) + 1;
-  }
+__device__ __forceinline__ void ring_copy(const SimDesc& D, int i, int j) {
+  const int NX = D.g.NX, NY = D.g.NY;
   const size_t c = (size_t)i * NY + j;
-  if (!(i >= 1 && j >= 1 && j <= NY - 2)) Uout[c] = Uin[c];
-  if (!(i >= 1 && i <= NX - 2 && j >= 1)) Vout[c] = Vin[c];
-  Pout[c] = Pin[c];
+  if (!(i >= 1 && j >= 1 && j <= NY - 2)) D.buf[B_U2][c] = D.buf[B_U][c];
+  if (!(i >= 1 && i <= NX - 2 && j >= 1)) D.buf[B_V2][c] = D.buf[B_V][c];
+  D.buf[B_T3][c] = D.buf[B_P][c];
+}
+__device__ __forceinline__ void ring_desc(const OneSim& m, const SimDesc*& D, int& t) {
+  D = &m.d;
+  t = blockIdx.x * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ void ring_desc(const ManySims& m, const SimDesc*& D, int& t) {
+  const RingItem it = m.rings[blockIdx.x];
+  D = &m.sims[it.sim];
+  t = it.blk * blockDim.x + threadIdx.x;
+}
+template <class M> __global__ void k_gs_ring(M m) {
+  const SimDesc* D;
+  int t;
+  ring_desc(m, D, t);
+  const int NX = D->g.NX, NY = D->g.NY;
+  if (t < NX) {
+    ring_copy(*D, t, 0);
+    ring_copy(*D, t, NY - 1);
+  }
+  if (t < NY) {
+    ring_copy(*D, 0, t);
+    ring_copy(*D, NX - 1, t);
+  }
 }
 
-// Zero the counters of both runs and the per-sweep maxima (graph-safe: no memset).
-__global__ void k_gs_init(int32_t* prog, int nprog, int32_t* ticket, uint32_t* maxdiv, int nmd, int32_t* ctrl) {
-  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < 2 * nprog; t += blockDim.x * gridDim.x) prog[t] = 0;
-  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nmd; t += blockDim.x * gridDim.x) maxdiv[t] = 0u;
+// Zero the counters of both runs, the per-sweep maxima and the re-run requests of every
+// simulation of the launch (graph-safe: no memset).
+__global__ void k_gs_init(int32_t* prog, int nprog, uint32_t* maxdiv, int nmd, int32_t* ctrl, int nsim,
+                          int32_t* ticket) {
+  const int stride = blockDim.x * gridDim.x;
+  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nprog; t += stride) prog[t] = 0;
+  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nmd; t += stride) maxdiv[t] = 0u;
+  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nsim; t += stride) ctrl[4 * t + 1] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ticket[0] = 0;
     ticket[1] = 0;
-    ctrl[1] = 0;
   }
 }
 
-// Early exit of solveSingleGrid (fluid.go:175-177): the first sweep whose max |div| is
-// below 1e-5 ends the solve; if that is not the last sweep, request a re-run of k*+1.
-__global__ void k_gs_decide(const uint32_t* maxdiv, int N, int early_exit, int32_t* ctrl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Early exit of solveSingleGrid (fluid.go:175-177), one thread per simulation: the first
+// sweep whose max |div| is below 1e-5 ends the solve; if that is not the last sweep,
+// request a re-run of k*+1 sweeps.
+template <class M> __global__ void k_gs_decide(M m, int nsim, int N) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nsim) return;
+  const SimDesc& D = sim_desc(m, e);
   int kept = N;
-  if (early_exit)
+  if (D.early_exit)
     for (int k = 0; k < N; k++)
-      if (__uint_as_float(maxdiv[k]) < 1e-5f) {
+      if (__uint_as_float(D.maxdiv[k]) < 1e-5f) {
         kept = k + 1;
         break;
       }
-  ctrl[1] = kept < N ? kept : 0;
-}
-
-template <int TK>
-static int launch_tk(Ctx* c, GsArgs& A, int grid) {
-  hipLaunchKernelGGL(k_gs_wave<TK>, dim3(grid), dim3(TK * LANES), 0, c->stream, A);
-  FL_HIP(hipGetLastError());
-  return FL_OK;
+  D.ctrl[1] = kept < N ? kept : 0;
 }
 
 static bool tile_empty_h(int g, int kb, int K, int TK, int NX) {
@@ -441,48 +450,72 @@ static bool tile_empty_h(int g, int kb, int K, int TK, int NX) {
   return rmax < 0 || rmin > NX - 1;
 }
 
-static int pick_tk(Ctx* c, int N) {
-  if (c->prm.block_sweeps == 8 || c->prm.block_sweeps == 16) return c->prm.block_sweeps;
+// Tile geometry of one simulation for N sweeps with TK sweeps per tile.
+struct TileGeom {
+  int G, KB, rows;
+};
+static TileGeom geom(int NX, int NY, int N, int tk) {
+  TileGeom t;
+  t.G = (NX + N - 1 + LANES - 1) / LANES;
+  t.KB = (N + tk - 1) / tk;
+  const int steps = NY + LANES - 1 + tk - 1;
+  t.rows = (steps + BLK - 1) / BLK * BLK;
+  return t;
+}
+static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, int NX, int N, int tk,
+                         const TileGeom& t) {
+  for (int g = 0; g < t.G; g++)
+    for (int kb = 0; kb < t.KB; kb++)
+      if (!tile_empty_h(g, kb, N, tk, NX)) {
+        long long key = ((long long)(LANES * g + kb * tk) << 32) | ((long long)sim << 12) | kb;
+        out.push_back({key, make_int2(sim, (g << 16) | kb)});
+      }
+}
+
+int pick_tk(const fl_params& P, int N) {
+  if (P.block_sweeps == 8 || P.block_sweeps == 16) return P.block_sweeps;
   return N >= 64 ? 16 : 8;
+}
+
+template <int TK> static int occupancy_grid(int dev, int ntiles, bool many) {
+  int per_cu = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1;
+  if (many)
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<TK, ManySims>, TK * LANES, 0);
+  else
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<TK, OneSim>, TK * LANES, 0);
+  return std::max(1, std::min(ntiles, std::max(1, per_cu) * prop.multiProcessorCount));
 }
 
 static int layout(Ctx* c, int N) {
   GsWork& W = c->gs;
-  const int tk = pick_tk(c, N);
+  const int tk = pick_tk(c->prm, N);
   if (W.n_alloc == N && W.tk == tk) return FL_OK;
+  FL_HIP(hipStreamSynchronize(c->stream));  // earlier solves may still use the old layout
   pressure_free(c);
+  const TileGeom t = geom(c->NX, c->NY, N, tk);
   W.tk = tk;
   W.n_alloc = N;
-  W.G = (c->NX + N - 1 + LANES - 1) / LANES;
-  W.KB = (N + tk - 1) / tk;
-  const int steps = c->NY + LANES - 1 + tk - 1;
-  W.rows = (steps + BLK - 1) / BLK * BLK;
-  std::vector<std::pair<long, int32_t>> order;
-  for (int g = 0; g < W.G; g++)
-    for (int kb = 0; kb < W.KB; kb++)
-      if (!tile_empty_h(g, kb, N, tk, c->NX)) order.push_back({(long)(LANES * g + kb * tk) * 65536 + kb, (g << 16) | kb});
-  std::sort(order.begin(), order.end());
-  std::vector<int32_t> tiles;
+  W.G = t.G;
+  W.KB = t.KB;
+  W.rows = t.rows;
+  std::vector<std::pair<long long, int2>> order;
+  append_tiles(order, 0, c->NX, N, tk, t);
+  std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::vector<int2> tiles;
   for (auto& o : order) tiles.push_back(o.second);
   W.ntiles = (int)tiles.size();
   const size_t nt = (size_t)W.G * W.KB;
   W.kout_elems = nt * W.rows * 3 * (LANES + 1);
   W.gout_elems = nt * W.rows * 3 * tk;
-  FL_HIP(hipMalloc(&W.d_tile, sizeof(int32_t) * std::max(1, W.ntiles)));
+  FL_HIP(hipMalloc(&W.d_tile, sizeof(int2) * std::max(1, W.ntiles)));
   FL_HIP(hipMalloc(&W.d_kout, sizeof(float) * W.kout_elems));
   FL_HIP(hipMalloc(&W.d_gout, sizeof(float) * W.gout_elems));
   FL_HIP(hipMalloc(&W.d_prog, sizeof(int32_t) * 2 * nt));
   FL_HIP(hipMalloc(&W.d_ticket, sizeof(int32_t) * 2));
-  FL_HIP(hipMemcpy(W.d_tile, tiles.data(), sizeof(int32_t) * W.ntiles, hipMemcpyHostToDevice));
-  int per_cu = 0, cus = 0;
-  hipDeviceProp_t prop;
-  FL_HIP(hipGetDeviceProperties(&prop, c->dev));
-  cus = prop.multiProcessorCount;
-  if (tk == 16)
-    FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<16>, 16 * LANES, 0));
-  else
-    FL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<8>, 8 * LANES, 0));
-  W.grid = std::max(1, std::min(W.ntiles, std::max(1, per_cu) * cus));
+  FL_HIP(hipMemcpy(W.d_tile, tiles.data(), sizeof(int2) * W.ntiles, hipMemcpyHostToDevice));
+  W.grid = tk == 16 ? occupancy_grid<16>(c->dev, W.ntiles, false) : occupancy_grid<8>(c->dev, W.ntiles, false);
   return FL_OK;
 }
 
@@ -492,47 +525,32 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   int rc = layout(c, N);
   if (rc) return rc;
   GsWork& W = c->gs;
-  if (rc) return rc;
   if (c->flags_dirty && (rc = launch_flags(c))) return rc;
+  SimDesc d = c->desc();
+  d.cp = cp;
+  OneSim m{d};
   const int nprog = W.G * W.KB;
-  const int nring = 2 * c->NX + 2 * (c->NY - 2);
-  hipLaunchKernelGGL(k_gs_ring, dim3((nring + 255) / 256), dim3(256), 0, c->stream, c->NX, c->NY, c->U, c->V, c->p,
-                     c->U2, c->V2, c->T3);
-  hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, nprog, W.d_ticket, c->d_maxdiv,
-                     2 * c->nmax, c->d_ctrl);
+  const int mr = std::max(c->NX, c->NY);
+  hipLaunchKernelGGL(k_gs_ring<OneSim>, dim3((mr + 255) / 256), dim3(256), 0, c->stream, m);
+  hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, 2 * nprog, c->d_maxdiv, 2 * c->nmax,
+                     c->d_ctrl, 1, W.d_ticket);
   FL_HIP(hipGetLastError());
-  GsArgs A;
-  A.NX = c->NX;
-  A.NY = c->NY;
-  A.G = W.G;
-  A.KB = W.KB;
-  A.rows = W.rows;
-  A.ntiles = W.ntiles;
-  A.K = N;
-  A.tiles = W.d_tile;
-  A.Uin = c->U;
-  A.Vin = c->V;
-  A.Pin = c->p;
-  A.Uout = c->U2;
-  A.Vout = c->V2;
-  A.Pout = c->T3;
-  A.fdiag = c->flags_diag;
-  A.omega = c->d_omega;
-  A.damping = c->prm.pressure_damping;
-  A.cp = cp;
-  A.kout = W.d_kout;
-  A.gout = W.d_gout;
-  A.ctrl = c->d_ctrl;
+  GsRun R;
+  R.tiles = (const int2*)W.d_tile;
+  R.ntiles = W.ntiles;
+  R.K = N;
+  R.omega = c->d_omega;
+  R.damping = c->prm.pressure_damping;
   for (int which = 0; which < 2; which++) {
-    A.which = which;
-    A.prog = W.d_prog + which * nprog;
-    A.ticket = W.d_ticket + which;
-    A.maxdiv = c->d_maxdiv + which * c->nmax;
-    rc = W.tk == 16 ? launch_tk<16>(c, A, W.grid) : launch_tk<8>(c, A, W.grid);
-    if (rc) return rc;
+    R.which = which;
+    R.ticket = W.d_ticket + which;
+    if (W.tk == 16)
+      hipLaunchKernelGGL((k_gs_wave<16, OneSim>), dim3(W.grid), dim3(16 * LANES), 0, c->stream, m, R);
+    else
+      hipLaunchKernelGGL((k_gs_wave<8, OneSim>), dim3(W.grid), dim3(8 * LANES), 0, c->stream, m, R);
+    FL_HIP(hipGetLastError());
     if (which == 0) {
-      hipLaunchKernelGGL(k_gs_decide, dim3(1), dim3(64), 0, c->stream, c->d_maxdiv, N, c->prm.early_exit,
-                         c->d_ctrl);
+      hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(64), 0, c->stream, m, 1, N);
       FL_HIP(hipGetLastError());
     }
   }
@@ -551,6 +569,110 @@ int pressure_free(Ctx* c) {
   if (W.d_prog) (void)hipFree(W.d_prog);
   if (W.d_ticket) (void)hipFree(W.d_ticket);
   W = GsWork{};
+  return FL_OK;
+}
+
+// ---- batched solve ------------------------------------------------------------------------
+// All members' tiles in one ticket list, merged by hyperplane start; one persistent launch
+// drains it (and one more for the members that early-exit). Results land in U2, V2, T3.
+int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
+  const int tk = pick_tk(B->prm, N);
+  auto fill = [&]() {
+    for (size_t e = 0; e < descs.size(); e++) {
+      const SimDesc& f = B->gs_fields[e];
+      SimDesc& d = descs[e];
+      d.G = f.G;
+      d.KB = f.KB;
+      d.rows = f.rows;
+      d.kout = f.kout;
+      d.gout = f.gout;
+      d.prog = f.prog;
+    }
+  };
+  if (B->gs_n == N && B->tk == tk && B->gs_fields.size() == descs.size()) {
+    fill();
+    return FL_OK;
+  }
+  FL_HIP(hipStreamSynchronize(B->stream));  // earlier solves may still use the old layout
+  if (B->d_tiles) (void)hipFree(B->d_tiles);
+  if (B->kout) (void)hipFree(B->kout);
+  if (B->gout) (void)hipFree(B->gout);
+  if (B->prog) (void)hipFree(B->prog);
+  if (B->ticket) (void)hipFree(B->ticket);
+  B->d_tiles = nullptr;
+  B->kout = B->gout = nullptr;
+  B->prog = B->ticket = nullptr;
+  std::vector<std::pair<long long, int2>> order;
+  size_t kelems = 0, gelems = 0, nprog = 0;
+  std::vector<TileGeom> tg(descs.size());
+  for (size_t e = 0; e < descs.size(); e++) {
+    tg[e] = geom(descs[e].g.NX, descs[e].g.NY, N, tk);
+    append_tiles(order, (int)e, descs[e].g.NX, N, tk, tg[e]);
+    const size_t nt = (size_t)tg[e].G * tg[e].KB;
+    kelems += nt * tg[e].rows * 3 * (LANES + 1);
+    gelems += nt * tg[e].rows * 3 * tk;
+    nprog += 2 * nt;
+  }
+  std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::vector<int2> tiles;
+  for (auto& o : order) tiles.push_back(o.second);
+  B->ntiles = (int)tiles.size();
+  FL_HIP(hipMalloc(&B->d_tiles, sizeof(int2) * std::max(1, B->ntiles)));
+  FL_HIP(hipMalloc(&B->kout, sizeof(float) * std::max<size_t>(kelems, 1)));
+  FL_HIP(hipMalloc(&B->gout, sizeof(float) * std::max<size_t>(gelems, 1)));
+  FL_HIP(hipMalloc(&B->prog, sizeof(int32_t) * std::max<size_t>(nprog, 1)));
+  FL_HIP(hipMalloc(&B->ticket, sizeof(int32_t) * 2));
+  FL_HIP(hipMemcpy(B->d_tiles, tiles.data(), sizeof(int2) * B->ntiles, hipMemcpyHostToDevice));
+  size_t ko = 0, go = 0, po = 0;
+  B->gs_fields.assign(descs.size(), SimDesc{});
+  for (size_t e = 0; e < descs.size(); e++) {
+    SimDesc& d = B->gs_fields[e];
+    d.G = tg[e].G;
+    d.KB = tg[e].KB;
+    d.rows = tg[e].rows;
+    d.kout = B->kout + ko;
+    d.gout = B->gout + go;
+    d.prog = B->prog + po;
+    const size_t nt = (size_t)d.G * d.KB;
+    ko += nt * d.rows * 3 * (LANES + 1);
+    go += nt * d.rows * 3 * tk;
+    po += 2 * nt;
+  }
+  B->tk = tk;
+  B->gs_n = N;
+  B->gs_nprog = (int)nprog;
+  B->grid = tk == 16 ? occupancy_grid<16>(B->dev, B->ntiles, true) : occupancy_grid<8>(B->dev, B->ntiles, true);
+  fill();
+  return FL_OK;
+}
+
+int batch_gs_solve(Batch* B, int N) {
+  ManySims m{B->d_sims, B->d_cells, B->d_rings};
+  const int E = (int)B->members.size();
+  hipLaunchKernelGGL(k_gs_ring<ManySims>, dim3(B->nring_items), dim3(256), 0, B->stream, m);
+  // progress counters, per-sweep maxima and re-run requests of all members are contiguous
+  hipLaunchKernelGGL(k_gs_init, dim3(256), dim3(256), 0, B->stream, B->prog, B->gs_nprog, B->maxdiv_pool,
+                     2 * B->nmax * E, B->ctrl_pool, E, B->ticket);
+  FL_HIP(hipGetLastError());
+  GsRun R;
+  R.tiles = B->d_tiles;
+  R.ntiles = B->ntiles;
+  R.K = N;
+  R.omega = B->d_omega;
+  R.damping = B->prm.pressure_damping;
+  for (int which = 0; which < 2; which++) {
+    R.which = which;
+    R.ticket = B->ticket + which;
+    if (B->tk == 16)
+      hipLaunchKernelGGL((k_gs_wave<16, ManySims>), dim3(B->grid), dim3(16 * LANES), 0, B->stream, m, R);
+    else
+      hipLaunchKernelGGL((k_gs_wave<8, ManySims>), dim3(B->grid), dim3(8 * LANES), 0, B->stream, m, R);
+    FL_HIP(hipGetLastError());
+    if (which == 0) {
+      hipLaunchKernelGGL(k_gs_decide<ManySims>, dim3((E + 63) / 64), dim3(64), 0, B->stream, m, E, N);
+      FL_HIP(hipGetLastError());
+    }
+  }
   return FL_OK;
 }
 

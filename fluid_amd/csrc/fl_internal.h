@@ -46,6 +46,45 @@ struct Source {
   float U, V;
 };
 
+// Buffer roles of one simulation (indices into SimDesc::buf).
+enum Buf { B_U = 0, B_V, B_NU, B_NV, B_P, B_S, B_M, B_NM, B_U2, B_V2, B_M2, B_T0, B_T1, B_T2, B_T3, NBUF };
+
+// Everything a kernel needs to know about one simulation: its grid, its buffers and,
+// for the Gauss-Seidel kernel, its tile layout and handoff buffers.
+struct SimDesc {
+  Grid g;
+  float* buf[NBUF];
+  uint8_t* flags;
+  uint8_t* diag;
+  int32_t* ctrl;      // [4]: [1] re-run sweeps, [2] invalid sample flag, [3] GS wait timeout
+  uint32_t* maxdiv;   // [2][nmax]
+  float* kout;        // GS K-stream handoff planes
+  float* gout;        // GS G-stream handoff planes
+  int32_t* prog;      // [2][G*KB] GS progress counters
+  int G, KB, rows, nmax;
+  float cp;           // density*h/dt of this step
+  int early_exit;
+};
+
+// A 64x4 block of cells of one simulation (batched cell kernels) and a 256-cell chunk
+// of one simulation's ring (batched ring kernels).
+struct CellItem {
+  int sim, bx, by, pad;
+};
+struct RingItem {
+  int sim, blk;
+};
+
+// Launch maps: one simulation passed by value, or many through a work-list.
+struct OneSim {
+  SimDesc d;
+};
+struct ManySims {
+  const SimDesc* sims;
+  const CellItem* cells;
+  const RingItem* rings;
+};
+
 // Pressure workspace of the temporal-blocked Gauss-Seidel kernel.
 struct GsWork {
   int tk = 0;              // sweeps per tile (waves per workgroup)
@@ -62,7 +101,11 @@ struct GsWork {
   size_t kout_elems = 0, gout_elems = 0;
 };
 
+struct Batch;
+
 struct Ctx {
+  Batch* batch = nullptr;  // set for members of a batch: buffers live in the batch's pools
+  int member = -1;
   int dev = 0;
   hipStream_t stream = nullptr;
   int NX = 0, NY = 0;
@@ -85,6 +128,8 @@ struct Ctx {
   uint32_t* d_maxdiv = nullptr;    // [2][nmax] float bits of per-sweep max |div| (main, rerun)
   float* d_omega = nullptr;        // [nmax] relaxation of each sweep
   int nmax = 0;
+  int omega_n = -1;                // sweep count and Relaxation the table holds
+  float omega_relax = 0.0f;
   uint32_t* d_red = nullptr;       // scratch for reductions
   float* d_xy = nullptr;           // scratch for batched sampling
   int xy_cap = 0;
@@ -106,6 +151,7 @@ struct Ctx {
   float jetU = 4.0f, jetSmoke = 1.0f;
   std::vector<Source> sources, sinks;
   bool src_dirty = true;
+  int src_gen = 0;  // bumped with every change of the source program
   SrcOp* d_src = nullptr;
   int32_t* d_src_seg = nullptr;
   int n_src_ops = 0, n_src_seg = 0;
@@ -121,10 +167,86 @@ struct Ctx {
   int last_n = 0;
   bool last_rollback_checked = true;
   int host_sweeps = -1;  // sweeps known on the host (hyperplane path), -1 if on device only
+
+  // Descriptor of this simulation with its current buffers.
+  SimDesc desc() const {
+    SimDesc d{};
+    d.g = grid;
+    float* b[NBUF] = {U, V, nU, nV, p, S, M, nM, U2, V2, M2, T0, T1, T2, T3};
+    for (int k = 0; k < NBUF; k++) d.buf[k] = b[k];
+    d.flags = flags;
+    d.diag = flags_diag;
+    d.ctrl = d_ctrl;
+    d.maxdiv = d_maxdiv;
+    d.kout = gs.d_kout;
+    d.gout = gs.d_gout;
+    d.prog = gs.d_prog;
+    d.G = gs.G;
+    d.KB = gs.KB;
+    d.rows = gs.rows;
+    d.nmax = nmax;
+    d.early_exit = prm.early_exit;
+    return d;
+  }
 };
+
+// A batch: E member simulations whose buffers are carved from shared pools, stepped
+// together with one launch per pass (fl_batch.hip).
+struct Batch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Ctx*> members;
+  float* pool[NBUF] = {};
+  uint8_t* flags_pool = nullptr;
+  uint8_t* diag_pool = nullptr;
+  int32_t* ctrl_pool = nullptr;
+  uint32_t* maxdiv_pool = nullptr;
+  size_t ncells = 0;
+  fl_params prm{};
+  int nmax = 0;
+  // work-lists
+  SimDesc* d_sims = nullptr;
+  CellItem* d_cells = nullptr;
+  RingItem* d_rings = nullptr;
+  int ncell_items = 0, nring_items = 0;
+  std::vector<SimDesc> descs;  // what d_sims holds
+  // batched Gauss-Seidel layout
+  int tk = 0, gs_n = -1, gs_nprog = 0;
+  int ntiles = 0, grid = 0;
+  std::vector<SimDesc> gs_fields;  // per member: G, KB, rows and its handoff/progress slices
+  int2* d_tiles = nullptr;
+  float* kout = nullptr;
+  float* gout = nullptr;
+  int32_t* prog = nullptr;
+  int32_t* ticket = nullptr;
+  float* d_omega = nullptr;
+  int omega_n = -1;
+  float omega_relax = 0.0f;
+  // per-step sources of all members
+  SrcOp* d_src = nullptr;
+  int32_t* d_src_seg = nullptr;
+  int32_t* d_src_sim = nullptr;
+  int n_src_seg = 0;
+  size_t src_cap = 0;
+  std::vector<int> src_gen_seen;
+  // captured step: replayed while dt, the parameters and every launch shape are unchanged
+  bool use_graph = true;
+  hipGraphExec_t graph = nullptr;
+  float graph_dt = -1.0f;
+  fl_params graph_prm{};
+  int graph_gen = 0, graph_key_gen = -1;  // bumped when a launch shape changes
+  int steps = 0;
+};
+
+// ---- host runtime shared by single contexts and batches (fl_capi.hip) ------------------
+int flush_edits(Ctx* c);
+void source_ops(Ctx* c, std::vector<SrcOp>& ops);
+void free_ctx(Ctx* c);
+fl_params default_params();
 
 // ---- launchers (fl_kernels.hip) ------------------------------------------------------
 int launch_fill(float* dst, float v, size_t n, hipStream_t s);
+int launch_zero_u8(uint8_t* dst, size_t n, hipStream_t s);
 int launch_copy(float* dst, const float* src, size_t n, hipStream_t s);
 int launch_flags(Ctx* c);
 int launch_copy_border(Ctx* c, float* dst, const float* src);
@@ -144,6 +266,23 @@ int launch_apply_sources(Ctx* c);
 int launch_max_reduce(Ctx* c, int kind, float* out);  // 0 MaxDivergence, 1 max(|U|+|V|)
 int launch_view(Ctx* c, int kind, float* out);        // 0 vorticity, 1 velocity magnitude
 int launch_sample_batch(Ctx* c, const float* xy, int n, float* uv);
+
+// ---- batched passes (fl_kernels.hip): one launch over all members of a batch -----------
+int batch_fill_role(Batch* B, int r, float v);
+int batch_copy_role(Batch* B, int rdst, int rsrc);
+int batch_flags(Batch* B);
+int batch_viscosity(Batch* B, float dt);
+int batch_conf_turb(Batch* B, float dt);
+int batch_borders(Batch* B);
+int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt);
+int batch_bfecc_velocity_corr(Batch* B, float dt);
+int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt);
+int batch_bfecc_smoke_corr(Batch* B, float dt);
+int batch_apply_sources(Batch* B);
+// batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
+int pick_tk(const fl_params& P, int N);
+int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
+int batch_gs_solve(Batch* B, int N);
 
 // ---- pressure (fl_pressure.hip) -------------------------------------------------------
 // Runs the single-grid projection: sweeps with the reference order on (U,V,p).

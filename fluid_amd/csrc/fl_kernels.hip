@@ -7,6 +7,11 @@
 // of the workloads (<= a few cells) these stay in L1/L2, so the passes run at the
 // stream rate of their field reads and writes. Arithmetic follows the reference
 // operation by operation (build with -ffp-contract=off).
+//
+// Every pass kernel is templated on its launch map: OneSim (one simulation passed by
+// value, grid of 64x4 blocks) or ManySims (a batch: each workgroup takes a (simulation,
+// block) item from a work-list), and addresses its fields by role (SimDesc::buf), so a
+// batch of uneven grids runs each pass as one launch.
 #include <hip/hip_runtime.h>
 
 #include "fl_internal.h"
@@ -17,13 +22,35 @@ static constexpr int BX = 64, BY = 4;
 
 static inline dim3 grid2d(int NX, int NY) { return dim3((NY + BX - 1) / BX, (NX + BY - 1) / BY); }
 
-#define CELL_PROLOGUE(g)                             \
-  const int j = blockIdx.x * BX + threadIdx.x;       \
-  const int i = blockIdx.y * BY + threadIdx.y;       \
-  if (i >= (g).NX || j >= (g).NY) return;            \
-  const int n = (g).NY;                              \
-  const int c = i * n + j;                           \
-  (void)n;
+// ---- launch maps --------------------------------------------------------------------------
+__device__ __forceinline__ bool cell_of(const OneSim& m, const SimDesc*& d, int& i, int& j) {
+  d = &m.d;
+  j = blockIdx.x * BX + threadIdx.x;
+  i = blockIdx.y * BY + threadIdx.y;
+  return i < d->g.NX && j < d->g.NY;
+}
+__device__ __forceinline__ bool cell_of(const ManySims& m, const SimDesc*& d, int& i, int& j) {
+  const CellItem it = m.cells[blockIdx.x];
+  d = &m.sims[it.sim];
+  j = it.bx * BX + threadIdx.x;
+  i = it.by * BY + threadIdx.y;
+  return i < d->g.NX && j < d->g.NY;
+}
+// Ring kernels: thread t of a simulation handles row i = t and column j = t.
+__device__ __forceinline__ void ring_of(const OneSim& m, const SimDesc*& d, int& t) {
+  d = &m.d;
+  t = blockIdx.x * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ void ring_of(const ManySims& m, const SimDesc*& d, int& t) {
+  const RingItem it = m.rings[blockIdx.x];
+  d = &m.sims[it.sim];
+  t = it.blk * blockDim.x + threadIdx.x;
+}
+
+template <class L> static void launch_cells(Ctx* c, const L& fn) {
+  OneSim m{c->desc()};
+  fn(m, grid2d(c->NX, c->NY), dim3(BX, BY), c->stream);
+}
 
 // ---------------------------------------------------------------------------------------
 __global__ void k_fill(float* __restrict__ d, float v, size_t n) {
@@ -56,80 +83,119 @@ int launch_copy(float* dst, const float* src, size_t n, hipStream_t s) {
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
+int launch_zero_u8(uint8_t* dst, size_t n, hipStream_t s) {
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+  if (blocks == 0) return FL_OK;
+  hipLaunchKernelGGL(k_fill_u8, dim3(blocks), dim3(256), 0, s, dst, n);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// Per-simulation fill and copy of a field role (batched steps copy results back instead of
+// swapping buffers, so a batch's descriptors stay fixed).
+template <class M> __global__ void k_fill_role(M m, int r, float v) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  d->buf[r][i * d->g.NY + j] = v;
+}
+template <class M> __global__ void k_copy_role(M m, int rdst, int rsrc) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const int c = i * d->g.NY + j;
+  d->buf[rdst][c] = d->buf[rsrc][c];
+}
 
 // ---- cell flags (from S, which holds exactly 0.0 or 1.0) ------------------------------
-__global__ void k_flags(Grid g, const float* __restrict__ S, uint8_t* __restrict__ flags,
-                        uint8_t* __restrict__ diag) {
-  CELL_PROLOGUE(g);
-  const int NX = g.NX, NY = g.NY;
+// The anti-diagonal copy is written only at grid cells; its other entries stay zero from
+// allocation.
+template <class M> __global__ void k_flags(M m) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const int NX = d->g.NX, NY = d->g.NY, n = NY, c = i * n + j;
+  const float* S = d->buf[B_S];
   uint32_t f = 0;
-  bool fluid = S[c] != 0.0f;
+  const bool fluid = S[c] != 0.0f;
   if (fluid) f |= CF_FLUID;
   if (i > 0 && S[c - n] != 0.0f) f |= CF_X0;
   if (i < NX - 1 && S[c + n] != 0.0f) f |= CF_X1;
   if (j > 0 && S[c - 1] != 0.0f) f |= CF_Y0;
   if (j < NY - 1 && S[c + 1] != 0.0f) f |= CF_Y1;
-  bool interior = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2;
+  const bool interior = i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2;
   if (interior) f |= CF_INTERIOR;
   if (interior && fluid && (f & (CF_X0 | CF_X1 | CF_Y0 | CF_Y1))) f |= CF_GS;
-  flags[c] = (uint8_t)f;
-  diag[(size_t)(i + j) * NX + i] = (uint8_t)f;
+  d->flags[c] = (uint8_t)f;
+  d->diag[(size_t)(i + j) * NX + i] = (uint8_t)f;
 }
 
 int launch_flags(Ctx* c) {
-  size_t nd = (size_t)(c->NX + c->NY - 1) * c->NX;
-  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)std::min<size_t>((nd + 255) / 256, 8192)), dim3(256), 0, c->stream,
-                     c->flags_diag, nd);
-  hipLaunchKernelGGL(k_flags, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->S, c->flags,
-                     c->flags_diag);
+  launch_cells(c, [](const OneSim& m, dim3 g, dim3 b, hipStream_t s) { hipLaunchKernelGGL(k_flags<OneSim>, g, b, 0, s, m); });
   FL_HIP(hipGetLastError());
   c->flags_dirty = false;
   return FL_OK;
 }
 
 // ---- copyBorder (fluid.go:436-446) ------------------------------------------------------
-__global__ void k_copy_border(Grid g, float* __restrict__ dst, const float* __restrict__ src) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = g.NY;
-  if (t < g.NX) {
+template <class M> __global__ void k_copy_border(M m, int rdst, int rsrc) {
+  const SimDesc* d;
+  int t;
+  ring_of(m, d, t);
+  const int n = d->g.NY, NX = d->g.NX, NY = d->g.NY;
+  float* dst = d->buf[rdst];
+  const float* src = d->buf[rsrc];
+  if (t < NX) {
     dst[t * n + 0] = src[t * n + 0];
-    dst[t * n + g.NY - 1] = src[t * n + g.NY - 1];
+    dst[t * n + NY - 1] = src[t * n + NY - 1];
   }
-  if (t < g.NY) {
+  if (t < NY) {
     dst[t] = src[t];
-    dst[(g.NX - 1) * n + t] = src[(g.NX - 1) * n + t];
+    dst[(NX - 1) * n + t] = src[(NX - 1) * n + t];
   }
 }
+static int role_of(Ctx* c, const float* p) {
+  const SimDesc d = c->desc();
+  for (int k = 0; k < NBUF; k++)
+    if (d.buf[k] == p) return k;
+  return -1;
+}
 int launch_copy_border(Ctx* c, float* dst, const float* src) {
-  int m = std::max(c->NX, c->NY);
-  hipLaunchKernelGGL(k_copy_border, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->grid, dst, src);
+  const int m = std::max(c->NX, c->NY);
+  OneSim map{c->desc()};
+  hipLaunchKernelGGL(k_copy_border<OneSim>, dim3((m + 255) / 256), dim3(256), 0, c->stream, map, role_of(c, dst),
+                     role_of(c, src));
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 // ---- applyViscosity (fluid.go:112-142): result lands in both newU/newV and U2/V2 ------
-__global__ void k_viscosity(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
-                            const float* __restrict__ V, float* __restrict__ nU, float* __restrict__ nV,
-                            float* __restrict__ Uo, float* __restrict__ Vo, float visc) {
-  CELL_PROLOGUE(g);
+template <class M> __global__ void k_viscosity(M m, float visc) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const int n = d->g.NY, c = i * n + j;
+  const float* U = d->buf[B_U];
+  const float* V = d->buf[B_V];
   float u = U[c], v = V[c];
   float ru = u, rv = v;
-  uint32_t f = flags[c];
+  const uint32_t f = d->flags[c];
   if ((f & CF_INTERIOR) && (f & CF_FLUID)) {
     float uLap = U[c - n] + U[c + n] + U[c - 1] + U[c + 1] - 4.0f * u;
     ru = u + visc * uLap;
     float vLap = V[c - n] + V[c + n] + V[c - 1] + V[c + 1] - 4.0f * v;
     rv = v + visc * vLap;
   }
-  nU[c] = ru;
-  Uo[c] = ru;
-  nV[c] = rv;
-  Vo[c] = rv;
+  d->buf[B_NU][c] = ru;
+  d->buf[B_U2][c] = ru;
+  d->buf[B_NV][c] = rv;
+  d->buf[B_V2][c] = rv;
 }
 int launch_viscosity(Ctx* c, float dt) {
-  float visc = c->prm.viscosity_diffusion * dt;
-  hipLaunchKernelGGL(k_viscosity, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, c->U, c->V,
-                     c->nU, c->nV, c->U2, c->V2, visc);
+  const float visc = c->prm.viscosity_diffusion * dt;
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) {
+    hipLaunchKernelGGL(k_viscosity<OneSim>, g, b, 0, s, m, visc);
+  });
   FL_HIP(hipGetLastError());
   std::swap(c->U, c->U2);
   std::swap(c->V, c->V2);
@@ -144,29 +210,38 @@ __device__ __forceinline__ float curl_at(const float* __restrict__ U, const floa
   return dvdx - dudy;
 }
 
-__global__ void k_curl(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
-                       const float* __restrict__ V, float* __restrict__ curl) {
-  CELL_PROLOGUE(g);
-  uint32_t f = flags[c];
-  curl[c] = ((f & CF_INTERIOR) && (f & CF_FLUID)) ? curl_at(U, V, c, n, g.h) : 0.0f;
+template <class M> __global__ void k_curl(M m, int rcurl) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const int n = d->g.NY, c = i * n + j;
+  const uint32_t f = d->flags[c];
+  d->buf[rcurl][c] =
+      ((f & CF_INTERIOR) && (f & CF_FLUID)) ? curl_at(d->buf[B_U], d->buf[B_V], c, n, d->g.h) : 0.0f;
 }
 int launch_curl(Ctx* c, float* curl) {
-  hipLaunchKernelGGL(k_curl, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, c->U, c->V,
-                     curl);
+  const int r = role_of(c, curl);
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) { hipLaunchKernelGGL(k_curl<OneSim>, g, b, 0, s, m, r); });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
-__global__ void k_conf_turb(Grid g, const uint8_t* __restrict__ flags, float* __restrict__ U, float* __restrict__ V,
-                            const float* __restrict__ curl, float conf, float dt, float turb, int do_conf,
-                            int do_turb) {
-  CELL_PROLOGUE(g);
-  uint32_t f = flags[c];
+template <class M>
+__global__ void k_conf_turb(M m, int rcurl, float conf, float dt, float turb, int do_conf, int do_turb) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const int n = d->g.NY, c = i * n + j;
+  const uint32_t f = d->flags[c];
   if (!((f & CF_INTERIOR) && (f & CF_FLUID))) return;
+  float* U = d->buf[B_U];
+  float* V = d->buf[B_V];
+  const float* curl = d->buf[rcurl];
+  const float h = d->g.h;
   float u = U[c], v = V[c];
   if (do_conf) {
-    float gx = (fabsf(curl[c + n]) - fabsf(curl[c - n])) * 0.5f / g.h;
-    float gy = (fabsf(curl[c + 1]) - fabsf(curl[c - 1])) * 0.5f / g.h;
+    float gx = (fabsf(curl[c + n]) - fabsf(curl[c - n])) * 0.5f / h;
+    float gy = (fabsf(curl[c + 1]) - fabsf(curl[c - 1])) * 0.5f / h;
     float mag = sqrtf(gx * gx + gy * gy) + 1e-5f;
     gx /= mag;
     gy /= mag;
@@ -190,17 +265,24 @@ __global__ void k_conf_turb(Grid g, const uint8_t* __restrict__ flags, float* __
   V[c] = v;
 }
 int launch_conf_turb(Ctx* c, const float* curl, float dt, int do_conf, int do_turb) {
-  float turb = c->prm.turbulence_strength * dt;
-  hipLaunchKernelGGL(k_conf_turb, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, c->U, c->V,
-                     curl, c->prm.confinement, dt, turb, do_conf, do_turb);
+  const float turb = c->prm.turbulence_strength * dt, conf = c->prm.confinement;
+  const int r = role_of(c, curl);
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) {
+    hipLaunchKernelGGL(k_conf_turb<OneSim>, g, b, 0, s, m, r, conf, dt, turb, do_conf, do_turb);
+  });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 // ---- handleBorders (fluid.go:236-289) -----------------------------------------------------
-__global__ void k_borders(Grid g, const float* __restrict__ S, float* __restrict__ U, float* __restrict__ V) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = g.NY, NX = g.NX, NY = g.NY;
+template <class M> __global__ void k_borders(M m) {
+  const SimDesc* d;
+  int t;
+  ring_of(m, d, t);
+  const int n = d->g.NY, NX = d->g.NX, NY = d->g.NY;
+  const float* S = d->buf[B_S];
+  float* U = d->buf[B_U];
+  float* V = d->buf[B_V];
   if (t < NX) {
     int i = t;
     if (S[i * n + 0] == 0.0f || S[i * n + 1] == 0.0f) {
@@ -238,8 +320,9 @@ __global__ void k_borders(Grid g, const float* __restrict__ S, float* __restrict
 }
 int launch_borders(Ctx* c) {
   if (c->NX < 3 || c->NY < 3) return FL_ERR_UNSUPPORTED;
-  int m = std::max(c->NX, c->NY);
-  hipLaunchKernelGGL(k_borders, dim3((m + 255) / 256), dim3(256), 0, c->stream, c->grid, c->S, c->U, c->V);
+  const int m = std::max(c->NX, c->NY);
+  OneSim map{c->desc()};
+  hipLaunchKernelGGL(k_borders<OneSim>, dim3((m + 255) / 256), dim3(256), 0, c->stream, map);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -253,14 +336,20 @@ __device__ __forceinline__ float avg_v(const float* __restrict__ V, int c, int n
 }
 
 // advectVelocity: copyBorder(newU,U), the advected faces, stale newU elsewhere; then U=newU.
-// Ut/Vt is the current velocity (traced and sampled); result goes to newU/newV and Uo/Vo.
-__global__ void k_advect_velocity(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ Ut,
-                                  const float* __restrict__ Vt, float* __restrict__ nU, float* __restrict__ nV,
-                                  float* __restrict__ Uo, float* __restrict__ Vo, float dt, int* __restrict__ bad) {
-  CELL_PROLOGUE(g);
-  const int NX = g.NX, NY = g.NY;
+// (Ut,Vt) is the current velocity (traced and sampled); the result goes to newU/newV and
+// to (Uo,Vo).
+template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int rUo, int rVo, float dt) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const Grid g = d->g;
+  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
+  const float* Ut = d->buf[rUt];
+  const float* Vt = d->buf[rVt];
+  float* nU = d->buf[B_NU];
+  float* nV = d->buf[B_NV];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = flags[c];
+  const uint32_t f = d->flags[c];
   int badl = 0;
   float ou, ov;
   if (i >= 1 && j >= 1 && j <= NY - 2 && (f & CF_FLUID) && (f & CF_X0)) {
@@ -286,14 +375,16 @@ __global__ void k_advect_velocity(Grid g, const uint8_t* __restrict__ flags, con
     ov = ring ? Vt[c] : nV[c];
   }
   nU[c] = ou;
-  Uo[c] = ou;
+  d->buf[rUo][c] = ou;
   nV[c] = ov;
-  Vo[c] = ov;
-  if (badl) atomicOr(bad, 1);
+  d->buf[rVo][c] = ov;
+  if (badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt) {
-  hipLaunchKernelGGL(k_advect_velocity, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, Ut,
-                     Vt, c->nU, c->nV, Uo, Vo, dt, c->d_ctrl + 2);
+  const int a = role_of(c, Ut), b = role_of(c, Vt), e = role_of(c, Uo), f = role_of(c, Vo);
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
+    hipLaunchKernelGGL(k_advect_velocity<OneSim>, g, bl, 0, s, m, a, b, e, f, dt);
+  });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -316,14 +407,19 @@ __device__ __forceinline__ float clamp_neighbors(float val, const float* __restr
 
 // advectVelocityBFECC passes 2 and 3 (fluid.go:935-987): backward trace with the
 // original velocity sampling the forward result, then correction and clamp.
-__global__ void k_bfecc_velocity_corr(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ Uo,
-                                      const float* __restrict__ Vo, const float* __restrict__ Uf,
-                                      const float* __restrict__ Vf, float* __restrict__ Uc, float* __restrict__ Vc,
-                                      float dt, int* __restrict__ bad) {
-  CELL_PROLOGUE(g);
-  const int NX = g.NX, NY = g.NY;
+template <class M>
+__global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, int rUc, int rVc, float dt) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const Grid g = d->g;
+  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
+  const float* Uo = d->buf[rUo];
+  const float* Vo = d->buf[rVo];
+  const float* Uf = d->buf[rUf];
+  const float* Vf = d->buf[rVf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = flags[c];
+  const uint32_t f = d->flags[c];
   int badl = 0;
   float bu, bv;
   if (i >= 1 && j >= 1 && j <= NY - 2 && (f & CF_FLUID) && (f & CF_X0)) {
@@ -356,27 +452,34 @@ __global__ void k_bfecc_velocity_corr(Grid g, const uint8_t* __restrict__ flags,
     cu = clamp_neighbors(cu, Uo, c, n);
     cv = clamp_neighbors(cv, Vo, c, n);
   }
-  Uc[c] = cu;
-  Vc[c] = cv;
-  if (badl) atomicOr(bad, 1);
+  d->buf[rUc][c] = cu;
+  d->buf[rVc][c] = cv;
+  if (badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd, const float* Vfwd,
                                float* Ucorr, float* Vcorr, float dt) {
-  hipLaunchKernelGGL(k_bfecc_velocity_corr, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags,
-                     Uorig, Vorig, Ufwd, Vfwd, Ucorr, Vcorr, dt, c->d_ctrl + 2);
+  const int a = role_of(c, Uorig), b = role_of(c, Vorig), e = role_of(c, Ufwd), f = role_of(c, Vfwd),
+            x = role_of(c, Ucorr), y = role_of(c, Vcorr);
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfecc_velocity_corr<OneSim>, g, bl, 0, s, m, a, b, e, f, x, y, dt);
+  });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 // advectSmoke (fluid.go:400-434) using the current (already advected) velocity.
-__global__ void k_advect_smoke(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
-                               const float* __restrict__ V, const float* __restrict__ Ms, float* __restrict__ nM,
-                               float* __restrict__ Mo, float dt, float smokeAdv, float visc,
-                               int* __restrict__ bad) {
-  CELL_PROLOGUE(g);
-  const int NX = g.NX, NY = g.NY;
+template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, float visc) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const Grid g = d->g;
+  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
+  const float* U = d->buf[B_U];
+  const float* V = d->buf[B_V];
+  const float* Ms = d->buf[rMs];
+  float* nM = d->buf[B_NM];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = flags[c];
+  const uint32_t f = d->flags[c];
   float out;
   if ((f & CF_INTERIOR) && (f & CF_FLUID)) {
     int badl = 0;
@@ -391,29 +494,36 @@ __global__ void k_advect_smoke(Grid g, const uint8_t* __restrict__ flags, const 
       val += sd * nb;
     }
     out = go_maxf(val, 0.0f);
-    if (badl) atomicOr(bad, 1);
+    if (badl) atomicOr(d->ctrl + 2, 1);
   } else {
     out = ring ? Ms[c] : nM[c];
   }
   nM[c] = out;
-  Mo[c] = out;
+  d->buf[rMo][c] = out;
 }
 int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt) {
-  hipLaunchKernelGGL(k_advect_smoke, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, c->U,
-                     c->V, Msrc, c->nM, Mo, dt, c->prm.smoke_advection, c->prm.viscosity_diffusion, c->d_ctrl + 2);
+  const int a = role_of(c, Msrc), b = role_of(c, Mo);
+  const float sa = c->prm.smoke_advection, visc = c->prm.viscosity_diffusion;
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
+    hipLaunchKernelGGL(k_advect_smoke<OneSim>, g, bl, 0, s, m, a, b, dt, sa, visc);
+  });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 // advectSmokeBFECC passes 2 and 3 (fluid.go:1013-1046).
-__global__ void k_bfecc_smoke_corr(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
-                                   const float* __restrict__ V, const float* __restrict__ Mo,
-                                   const float* __restrict__ Mf, float* __restrict__ Mc, float dt, float smokeAdv,
-                                   int* __restrict__ bad) {
-  CELL_PROLOGUE(g);
-  const int NX = g.NX, NY = g.NY;
+template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int rMc, float dt, float smokeAdv) {
+  const SimDesc* d;
+  int i, j;
+  if (!cell_of(m, d, i, j)) return;
+  const Grid g = d->g;
+  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
+  const float* U = d->buf[B_U];
+  const float* V = d->buf[B_V];
+  const float* Mo = d->buf[rMo];
+  const float* Mf = d->buf[rMf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = flags[c];
+  const uint32_t f = d->flags[c];
   float om = Mo[c];
   float cm = om;
   if (!ring) {
@@ -425,17 +535,84 @@ __global__ void k_bfecc_smoke_corr(Grid g, const uint8_t* __restrict__ flags, co
       float x = (float)i * g.h + g.hs + dt * u;
       float y = (float)j * g.h + g.hs + dt * v;
       bm = sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
-      if (badl) atomicOr(bad, 1);
+      if (badl) atomicOr(d->ctrl + 2, 1);
     }
     cm = om - (bm - om) * 0.5f;
     cm = clamp_neighbors(cm, Mo, c, n);
     if (cm < 0.0f) cm = 0.0f;
   }
-  Mc[c] = cm;
+  d->buf[rMc][c] = cm;
 }
 int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt) {
-  hipLaunchKernelGGL(k_bfecc_smoke_corr, grid2d(c->NX, c->NY), dim3(BX, BY), 0, c->stream, c->grid, c->flags, c->U,
-                     c->V, Morig, Mfwd, Mcorr, dt, c->prm.smoke_advection, c->d_ctrl + 2);
+  const int a = role_of(c, Morig), b = role_of(c, Mfwd), e = role_of(c, Mcorr);
+  const float sa = c->prm.smoke_advection;
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfecc_smoke_corr<OneSim>, g, bl, 0, s, m, a, b, e, dt, sa);
+  });
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+// ---- batched passes: one launch over the work-list of all members ------------------------
+static ManySims many(Batch* B) { return ManySims{B->d_sims, B->d_cells, B->d_rings}; }
+#define FL_BATCH_CELLS(B) dim3((B)->ncell_items), dim3(BX, BY), 0, (B)->stream
+#define FL_BATCH_RING(B) dim3((B)->nring_items), dim3(256), 0, (B)->stream
+
+int batch_fill_role(Batch* B, int r, float v) {
+  hipLaunchKernelGGL(k_fill_role<ManySims>, FL_BATCH_CELLS(B), many(B), r, v);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_copy_role(Batch* B, int rdst, int rsrc) {
+  hipLaunchKernelGGL(k_copy_role<ManySims>, FL_BATCH_CELLS(B), many(B), rdst, rsrc);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_flags(Batch* B) {
+  hipLaunchKernelGGL(k_flags<ManySims>, FL_BATCH_CELLS(B), many(B));
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_viscosity(Batch* B, float dt) {
+  hipLaunchKernelGGL(k_viscosity<ManySims>, FL_BATCH_CELLS(B), many(B), B->prm.viscosity_diffusion * dt);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_conf_turb(Batch* B, float dt) {
+  const fl_params& P = B->prm;
+  const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
+  if (conf) hipLaunchKernelGGL(k_curl<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_T0);
+  if (conf || turb)
+    hipLaunchKernelGGL(k_conf_turb<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_T0, P.confinement, dt,
+                       P.turbulence_strength * dt, (int)conf, (int)turb);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_borders(Batch* B) {
+  hipLaunchKernelGGL(k_borders<ManySims>, FL_BATCH_RING(B), many(B));
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt) {
+  hipLaunchKernelGGL(k_advect_velocity<ManySims>, FL_BATCH_CELLS(B), many(B), rUt, rVt, rUo, rVo, dt);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_bfecc_velocity_corr(Batch* B, float dt) {
+  hipLaunchKernelGGL(k_bfecc_velocity_corr<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_U, (int)B_V, (int)B_U2,
+                     (int)B_V2, (int)B_T0, (int)B_T1, dt);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt) {
+  hipLaunchKernelGGL(k_advect_smoke<ManySims>, FL_BATCH_CELLS(B), many(B), rMs, rMo, dt, B->prm.smoke_advection,
+                     B->prm.viscosity_diffusion);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_bfecc_smoke_corr(Batch* B, float dt) {
+  hipLaunchKernelGGL(k_bfecc_smoke_corr<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_M, (int)B_M2, (int)B_T2, dt,
+                     B->prm.smoke_advection);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -478,15 +655,13 @@ int launch_apply_edits(Ctx* c, int nseg) {
 }
 
 // Per-step sources: one thread per cell, ops in reference order, IsSolid read at apply time.
-__global__ void k_apply_sources(const SrcOp* __restrict__ ops, const int32_t* __restrict__ seg, int nseg,
-                                const float* __restrict__ S, float* __restrict__ U, float* __restrict__ V,
-                                float* __restrict__ M) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nseg) return;
-  int b = seg[s], e = seg[s + 1];
-  int cell = ops[b].cell;
+// `sim` (batched program) names the member each segment belongs to.
+__device__ __forceinline__ void apply_source_seg(const SrcOp* __restrict__ ops, int b, int e,
+                                                 const float* __restrict__ S, float* __restrict__ U,
+                                                 float* __restrict__ V, float* __restrict__ M) {
+  const int cell = ops[b].cell;
   float u = U[cell], v = V[cell], m = M[cell];
-  bool fluid = S[cell] != 0.0f;
+  const bool fluid = S[cell] != 0.0f;
   for (int k = b; k < e; k++) {
     SrcOp op = ops[k];
     if (op.check_solid && !fluid) continue;
@@ -501,10 +676,32 @@ __global__ void k_apply_sources(const SrcOp* __restrict__ ops, const int32_t* __
   V[cell] = v;
   M[cell] = m;
 }
+__global__ void k_apply_sources(const SrcOp* __restrict__ ops, const int32_t* __restrict__ seg, int nseg,
+                                const float* __restrict__ S, float* __restrict__ U, float* __restrict__ V,
+                                float* __restrict__ M) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  apply_source_seg(ops, seg[s], seg[s + 1], S, U, V, M);
+}
+__global__ void k_apply_sources_batch(const SimDesc* __restrict__ sims, const SrcOp* __restrict__ ops,
+                                      const int32_t* __restrict__ seg, const int32_t* __restrict__ segsim,
+                                      int nseg) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const SimDesc& d = sims[segsim[s]];
+  apply_source_seg(ops, seg[s], seg[s + 1], d.buf[B_S], d.buf[B_U], d.buf[B_V], d.buf[B_M]);
+}
 int launch_apply_sources(Ctx* c) {
   if (c->n_src_seg == 0) return FL_OK;
   hipLaunchKernelGGL(k_apply_sources, dim3((c->n_src_seg + 255) / 256), dim3(256), 0, c->stream, c->d_src,
                      c->d_src_seg, c->n_src_seg, c->S, c->U, c->V, c->M);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+int batch_apply_sources(Batch* B) {
+  if (B->n_src_seg == 0) return FL_OK;
+  hipLaunchKernelGGL(k_apply_sources_batch, dim3((B->n_src_seg + 255) / 256), dim3(256), 0, B->stream, B->d_sims,
+                     B->d_src, B->d_src_seg, B->d_src_sim, B->n_src_seg);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -551,7 +748,10 @@ int launch_max_reduce(Ctx* c, int kind, float* out) {
 
 __global__ void k_view(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
                        const float* __restrict__ V, int kind, float* __restrict__ out) {
-  CELL_PROLOGUE(g);
+  const int j = blockIdx.x * BX + threadIdx.x;
+  const int i = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.NX || j >= g.NY) return;
+  const int n = g.NY, c = i * n + j;
   uint32_t f = flags[c];
   float r = 0.0f;
   if ((f & CF_INTERIOR) && (f & CF_FLUID)) {

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 
 #include "fl_internal.h"
 
@@ -87,6 +88,10 @@ static int hyperplane_run(Ctx* c, int K, float cp, uint32_t* maxdiv) {
 }
 
 static int ensure_omega(Ctx* c, int numIters) {
+  if (numIters > c->nmax && c->batch) {
+    set_error("a batch member supports at most " + std::to_string(c->nmax) + " sweeps");
+    return FL_ERR_UNSUPPORTED;
+  }
   if (numIters > c->nmax) {
     if (c->d_omega) FL_HIP(hipFree(c->d_omega));
     if (c->d_maxdiv) FL_HIP(hipFree(c->d_maxdiv));
@@ -97,8 +102,11 @@ static int ensure_omega(Ctx* c, int numIters) {
     FL_HIP(hipMalloc(&c->d_maxdiv, sizeof(uint32_t) * 2 * cap));
     FL_HIP(hipMemsetAsync(c->d_maxdiv, 0, sizeof(uint32_t) * 2 * cap, c->stream));
     c->nmax = cap;
+    c->omega_n = -1;
   }
-  // Relaxation schedule of solveSingleGrid (fluid.go:162-170), in float32.
+  // Relaxation schedule of solveSingleGrid (fluid.go:162-170), in float32; uploaded only
+  // when the sweep count or Relaxation changes, so a steady-state step never syncs.
+  if (c->omega_n == numIters && c->omega_relax == c->prm.relaxation) return FL_OK;
   std::vector<float> om(numIters);
   const float init = c->prm.relaxation, minR = 1.2f;
   for (int k = 0; k < numIters; k++) {
@@ -108,6 +116,8 @@ static int ensure_omega(Ctx* c, int numIters) {
   FL_HIP(hipMemcpyAsync(c->d_omega, om.data(), sizeof(float) * numIters, hipMemcpyHostToDevice, c->stream));
   // The host vector must outlive the async copy.
   FL_HIP(hipStreamSynchronize(c->stream));
+  c->omega_n = numIters;
+  c->omega_relax = c->prm.relaxation;
   return FL_OK;
 }
 
@@ -128,6 +138,7 @@ int pressure_solve(Ctx* c, int numIters, float dt) {
   if (c->prm.early_exit) {
     FL_HIP(hipMemcpyAsync(c->U2, c->U, bytes, hipMemcpyDeviceToDevice, c->stream));
     FL_HIP(hipMemcpyAsync(c->V2, c->V, bytes, hipMemcpyDeviceToDevice, c->stream));
+    FL_HIP(hipMemcpyAsync(c->T3, c->p, bytes, hipMemcpyDeviceToDevice, c->stream));
   }
   rc = hyperplane_run(c, numIters, cp, c->d_maxdiv);
   if (rc) return rc;
@@ -147,8 +158,7 @@ int pressure_solve(Ctx* c, int numIters, float dt) {
     if (kept < numIters) {
       FL_HIP(hipMemcpyAsync(c->U, c->U2, bytes, hipMemcpyDeviceToDevice, c->stream));
       FL_HIP(hipMemcpyAsync(c->V, c->V2, bytes, hipMemcpyDeviceToDevice, c->stream));
-      rc = launch_fill(c->p, 0.0f, c->N, c->stream);
-      if (rc) return rc;
+      FL_HIP(hipMemcpyAsync(c->p, c->T3, bytes, hipMemcpyDeviceToDevice, c->stream));
       rc = hyperplane_run(c, kept, cp, c->d_maxdiv + c->nmax);
       if (rc) return rc;
     }

@@ -67,32 +67,60 @@ class VectorField:
         return float(self.valuesU.reshape(-1)[k]), float(self.valuesV.reshape(-1)[k]), None
 
 
+def _init_knobs(o):
+    """The exported knobs of Fluid (fluid.go:27-39) with New's defaults (fluid.go:59-66)."""
+    o.Confinement = 0.0
+    o.ViscosityDiffusion = 0.0
+    o.PressureDamping = 1.0
+    o.TurbulenceStrength = 0.02
+    o.SmokeAdvection = 1.0
+    o.UseMultigrid = False
+    o.MultigridLevels = 2
+    o.UseBFECC = False
+    # Additions: the reference hard-codes 8 sweeps and the early exit (fluid.go:81,175).
+    o.PressureIters = 8
+    o.EarlyExit = True
+    o.Relaxation = None  # None: follow the module-level Relaxation like the Go package var
+    o.Solver = 0         # 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel
+    o.BlockSweeps = 0
+
+
+def _params_of(o):
+    p = FlParams()
+    p.confinement = o.Confinement
+    p.viscosity_diffusion = o.ViscosityDiffusion
+    p.pressure_damping = o.PressureDamping
+    p.turbulence_strength = o.TurbulenceStrength
+    p.smoke_advection = o.SmokeAdvection
+    p.relaxation = Relaxation if o.Relaxation is None else o.Relaxation
+    p.use_multigrid = int(bool(o.UseMultigrid))
+    p.multigrid_levels = int(o.MultigridLevels)
+    p.use_bfecc = int(bool(o.UseBFECC))
+    p.pressure_iters = int(o.PressureIters)
+    p.early_exit = int(bool(o.EarlyExit))
+    p.solver = int(o.Solver)
+    p.block_sweeps = int(o.BlockSweeps)
+    return p
+
+
 class Fluid:
     """Fluid (fluid.go:11-40) backed by device buffers on one MI355X."""
 
-    def __init__(self, density, width, height, h):
+    def __init__(self, density, width, height, h, _member=None):
         self._L = _lib.load()
-        ctx = ctypes.c_void_p()
-        check(self._L.fl_create(density, width, height, h, ctypes.byref(ctx)))
+        if _member is None:
+            ctx = ctypes.c_void_p()
+            check(self._L.fl_create(density, width, height, h, ctypes.byref(ctx)))
+            self._owned = True
+        else:
+            ctx = _member
+            self._owned = False  # a batch member belongs to its batch
         self._ctx = ctx
         nx, ny = ctypes.c_int(), ctypes.c_int()
         check(self._L.fl_dims(ctx, ctypes.byref(nx), ctypes.byref(ny)))
         self.NumX, self.NumY = nx.value, ny.value
         self._hval = float(np.float32(h))
-        self.Confinement = 0.0
-        self.ViscosityDiffusion = 0.0
-        self.PressureDamping = 1.0
-        self.TurbulenceStrength = 0.02
-        self.SmokeAdvection = 1.0
-        self.UseMultigrid = False
-        self.MultigridLevels = 2
-        self.UseBFECC = False
-        # Additions: the reference hard-codes 8 sweeps and the early exit (fluid.go:81,175).
-        self.PressureIters = 8
-        self.EarlyExit = True
-        self.Relaxation = None  # None: follow the module-level Relaxation like the Go package var
-        self.Solver = 0         # 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel
-        self.BlockSweeps = 0
+        _init_knobs(self)
 
     @classmethod
     def New(cls, density, width, height, h):
@@ -101,7 +129,8 @@ class Fluid:
 
     def close(self):
         if getattr(self, "_ctx", None):
-            self._L.fl_destroy(self._ctx)
+            if self._owned:
+                self._L.fl_destroy(self._ctx)
             self._ctx = None
 
     def __del__(self):
@@ -114,21 +143,17 @@ class Fluid:
         return self._hval
 
     def _push(self):
-        p = FlParams()
-        p.confinement = self.Confinement
-        p.viscosity_diffusion = self.ViscosityDiffusion
-        p.pressure_damping = self.PressureDamping
-        p.turbulence_strength = self.TurbulenceStrength
-        p.smoke_advection = self.SmokeAdvection
-        p.relaxation = Relaxation if self.Relaxation is None else self.Relaxation
-        p.use_multigrid = int(bool(self.UseMultigrid))
-        p.multigrid_levels = int(self.MultigridLevels)
-        p.use_bfecc = int(bool(self.UseBFECC))
-        p.pressure_iters = int(self.PressureIters)
-        p.early_exit = int(bool(self.EarlyExit))
-        p.solver = int(self.Solver)
-        p.block_sweeps = int(self.BlockSweeps)
+        p = _params_of(self)
         check(self._L.fl_set_params(self._ctx, ctypes.byref(p)))
+
+    def autotune(self, dt):
+        """Time the candidate Gauss-Seidel tile depths on the current state; keep the fastest."""
+        self._push()
+        tk, ms = ctypes.c_int(), ctypes.c_float()
+        check(self._L.fl_autotune(self._ctx, dt, ctypes.byref(tk), ctypes.byref(ms)))
+        if tk.value:
+            self.BlockSweeps = tk.value
+        return tk.value, ms.value
 
     # ---- whole-field access ---------------------------------------------------------
     def get(self, name):
@@ -317,6 +342,70 @@ class Fluid:
 def New(density, width, height, h):
     """New (fluid.go:42)."""
     return Fluid(density, width, height, h)
+
+
+class FluidBatch:
+    """An ensemble of independent Fluids of uneven sizes advanced together.
+
+    In the reference an ensemble is a loop of Simulate calls, one per Fluid (fluid.go:79,
+    main.go:231-249 per frame). Here `update(dt)` steps every member with one launch per
+    pass and one persistent pressure launch over all members' tiles, bit-identically to
+    calling `update(dt)` (sources, jet, Simulate) on each member alone with the batch's
+    knobs. `members[e]` is a Fluid view for setup, edits, scenes and downloads.
+    """
+
+    def __init__(self, density, sizes, h):
+        self._L = _lib.load()
+        n = len(sizes)
+        ws = (ctypes.c_int * n)(*[int(w) for w, _ in sizes])
+        hs = (ctypes.c_int * n)(*[int(hh) for _, hh in sizes])
+        b = ctypes.c_void_p()
+        check(self._L.fl_batch_create(n, ws, hs, density, h, ctypes.byref(b)))
+        self._b = b
+        self.members = []
+        for e in range(n):
+            m = ctypes.c_void_p()
+            check(self._L.fl_batch_member(b, e, ctypes.byref(m)))
+            self.members.append(Fluid(density, sizes[e][0], sizes[e][1], h, _member=m))
+        _init_knobs(self)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for m in self.members:
+                m._ctx = None
+            self._L.fl_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.members)
+
+    def _push(self):
+        p = _params_of(self)
+        check(self._L.fl_batch_set_params(self._b, ctypes.byref(p)))
+
+    def update(self, dt):
+        self._push()
+        check(self._L.fl_batch_update(self._b, dt))
+
+    def sync(self):
+        check(self._L.fl_batch_sync(self._b))
+
+    def set_graph(self, on):
+        check(self._L.fl_batch_set_graph(self._b, int(bool(on))))
+
+    def autotune(self, dt):
+        self._push()
+        tk, ms = ctypes.c_int(), ctypes.c_float()
+        check(self._L.fl_batch_autotune(self._b, dt, ctypes.byref(tk), ctypes.byref(ms)))
+        if tk.value:
+            self.BlockSweeps = tk.value
+        return tk.value, ms.value
 
 
 def device_count():

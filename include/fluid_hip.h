@@ -173,6 +173,33 @@ int fl_scene_update(fl_ctx* ctx, float dt);
 int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid); /* drawLine main.go:809 */
 int fl_scene_smoke_brush(fl_ctx* ctx, int ci, int cj, int radius, float strength); /* main.go:453 */
 
+/* Gauss-Seidel tile depth autotuner for one context: times the candidate depths of
+ * the temporal-blocked solve on the current state (which it leaves unchanged) and
+ * keeps the fastest in params.block_sweeps. The reference has no counterpart (its
+ * solve is a serial loop, fluid.go:188-234). */
+int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms);
+
+/* Ensembles. The reference steps one Fluid per call (Simulate, fluid.go:79); an
+ * ensemble of independent simulations of uneven sizes is a loop over Fluids there
+ * (main.go:245 per frame). A batch advances all of them with one launch per pass.
+ * Each member is an ordinary fl_ctx (fl_batch_member) for setup, edits, scenes, views
+ * and downloads; it must not be fl_destroy'd (the batch owns it). fl_batch_update
+ * applies every member's per-step sources and jet (as fl_scene_update does) and then
+ * Simulate(dt) with the batch's parameters, identically to stepping each member on its
+ * own. The step is captured as a HIP graph and replayed while dt, the parameters and
+ * the members' source programs are unchanged (fl_batch_set_graph(b, 0) turns this off). */
+typedef struct fl_batch fl_batch;
+int fl_batch_create(int n, const int* widths, const int* heights, float density, float h, fl_batch** out);
+int fl_batch_destroy(fl_batch* b);
+int fl_batch_size(fl_batch* b, int* n);
+int fl_batch_member(fl_batch* b, int index, fl_ctx** out);
+int fl_batch_get_params(fl_batch* b, fl_params* out);
+int fl_batch_set_params(fl_batch* b, const fl_params* in); /* pressure_iters <= 1024 */
+int fl_batch_update(fl_batch* b, float dt);
+int fl_batch_sync(fl_batch* b);
+int fl_batch_set_graph(fl_batch* b, int on);
+int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,12 +78,129 @@ def cpu_baseline(kind, size, iters, threads):
                                                                    sample_size, dt)}
 
 
+def ensemble_step(target, specs, step, members):
+    """One step of the ensemble workload: the step's Gaussian impulses, then Update."""
+    from fluid_amd import configs as C
+    for (e, cx, cy, fx, fy, sg) in C.ensemble_impulses(step, specs):
+        members[e].ApplyForceGaussian(cx, cy, fx, fy, sg)
+    for t in target:
+        t.update(C.DT)
+
+
+def bench_ensemble(args):
+    """The batched ensemble: 256 members of seeded sizes 64..1024, presets, wall strokes,
+    Gaussian impulses every 10 steps, 60 sweeps, SL, stepped by one FluidBatch (one launch
+    per pass, one persistent pressure launch, graph replay, autotuned tile depth), against
+    the same members stepped as 256 separate Fluids (one set of launches per grid)."""
+    from fluid_amd import Fluid, FluidBatch
+    from fluid_amd import configs as C
+    specs = C.ensemble_specs(args.members, seed=4)
+    cells = sum(s["w"] * s["h"] for s in specs)
+    print("bench: ensemble of %d members, %.1f M cells, %d sweeps" % (len(specs), cells / 1e6, args.iters),
+          file=sys.stderr, flush=True)
+    b = FluidBatch(C.DENSITY, [(s["w"], s["h"]) for s in specs], C.H)
+    for m, s in zip(b.members, specs):
+        C.ensemble_member_setup(m, s, args.iters)
+    C.ensemble_knobs(b, args.iters)
+    b.update(C.DT)
+    b.sync()
+    tk, solve_ms = b.autotune(C.DT)
+    print("bench: autotuned tile depth %d (solve %.2f ms)" % (tk, solve_ms), file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        ensemble_step([b], specs, 1 + w, b.members)
+    b.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ensemble_step([b], specs, 1 + args.warmup + k, b.members)
+    b.sync()
+    wall = time.perf_counter() - t0
+    ms_batch = 1e3 * wall / args.steps
+    sweeps = [m.stats()["sweeps_run"] for m in b.members]
+    solve_cell_sweeps = sum(s["w"] * s["h"] * k for s, k in zip(specs, sweeps))
+    b_alg_solve = 28.0 * solve_cell_sweeps
+    achieved = b_alg_solve / (solve_ms * 1e-3) / 1e9
+    step_bytes = sum(alg_bytes_per_cell(k, False, False, True, False) * s["w"] * s["h"] for s, k in zip(specs, sweeps))
+    b.close()
+
+    per_grid = None
+    if not args.no_per_grid:
+        print("bench: per-grid launches (%d separate Fluids)" % len(specs), file=sys.stderr, flush=True)
+        solo = [Fluid(C.DENSITY, s["w"], s["h"], C.H) for s in specs]
+        for f, s in zip(solo, specs):
+            C.ensemble_member_setup(f, s, args.iters)
+            f.BlockSweeps = tk
+        for f in solo:
+            f.update(C.DT)
+        for w in range(args.warmup):
+            ensemble_step(solo, specs, 1 + w, solo)
+        for f in solo:
+            f.sync()
+        ns = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        for k in range(ns):
+            ensemble_step(solo, specs, 1 + args.warmup + k, solo)
+        for f in solo:
+            f.sync()
+        ms_grid = 1e3 * (time.perf_counter() - t0) / ns
+        per_grid = {"ms_per_step": ms_grid, "value": cells / (ms_grid * 1e-3), "steps": ns}
+        for f in solo:
+            f.close()
+
+    out = {
+        "metric": METRIC,
+        "value": cells / (ms_batch * 1e-3),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "ensemble of %d members, sides U{64,96,..,1024} (seed 4), presets jet/cavity/obstacle, "
+                               "4 wall strokes each, Gaussian impulses every 10 steps, %d GS sweeps, SL, dt 0.1"
+                               % (len(specs), args.iters),
+                   "members": len(specs), "cells": cells, "pressure_sweeps": args.iters, "block_sweeps": tk},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "batched pressure solve (28 B x sum of member cells x sweeps per launch)"},
+        "step_achieved_GBs": step_bytes / (ms_batch * 1e-3) / 1e9,
+        "step_frac_hbm": step_bytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "per_grid": per_grid,
+        "batched_vs_per_grid": (per_grid["ms_per_step"] / ms_batch) if per_grid else None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import OracleFluid
+        print("bench: CPU baseline (float32 oracle, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
+        done, t0 = 0, time.perf_counter()
+        stepped = []
+        for s in specs:
+            o = OracleFluid(C.DENSITY, s["w"], s["h"], C.H, threads=args.cpu_threads)
+            C.ensemble_member_setup(o, s, args.iters)
+            t1 = time.perf_counter()
+            o.update(C.DT)
+            done += time.perf_counter() - t1
+            stepped.append(s["w"] * s["h"])
+            if done > 10.0:
+                break
+        out["cpu_baseline"] = {"value": sum(stepped) / done, "unit": "cell-steps/s", "cores": args.cpu_threads,
+                               "kind": "port", "sample": "float32 oracle, one step of each of the first %d ensemble "
+                                                         "members (%.1f M cells), %.2f s" % (
+                                                             len(stepped), sum(stepped) / 1e6, done)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman"])
+    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble"])
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--no-per-grid", action="store_true")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--solver", type=int, default=0)
@@ -100,6 +217,11 @@ def main():
 
     from fluid_amd import Fluid
     from fluid_amd import configs as C
+
+    if args.config == "ensemble":
+        if args.iters == 200:
+            args.iters = 60
+        return bench_ensemble(args)
 
     if args.calibrate:
         import torch

@@ -99,6 +99,7 @@ SIGNATURES = {
     "fl_scene_update": (_i, [_vp, _f]),
     "fl_scene_draw_line": (_i, [_vp, _i, _i, _i, _i, _i]),
     "fl_scene_smoke_brush": (_i, [_vp, _i, _i, _i, _f]),
+    "fl_apply_force_gaussian": (_i, [_vp, _i, _i, _f, _f, _f]),
     "fl_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
     "fl_batch_create": (_i, [_i, _ip, _ip, _f, _f, ctypes.POINTER(_vp)]),
     "fl_batch_destroy": (_i, [_vp]),

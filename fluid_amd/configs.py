@@ -154,3 +154,73 @@ def gaussian_blobs(nx, ny, seed=5, count=6, sigma=20.0):
         cy = 1 + rng.uniform() * (ny - 2)
         T += np.exp(-((ii - cx) ** 2 + (jj - cy) ** 2) / (2 * sigma * sigma))
     return T.astype(np.float32)
+
+
+# ---- the batched ensemble --------------------------------------------------------------
+ENSEMBLE_SIDES = tuple(range(64, 1025, 32))  # {64, 96, ..., 1024}
+ENSEMBLE_PRESETS = ("jet", "cavity", "obstacle")
+
+
+def ensemble_specs(n=256, seed=4, sides=ENSEMBLE_SIDES, strokes=4):
+    """Member descriptions of the ensemble: width and height uniform over `sides`, a
+    preset uniform over jet / cavity / obstacle, and `strokes` solid Bresenham strokes
+    (drawLine, main.go:809) from a uniform interior cell to a point at most a quarter of
+    the shorter side away. Returns [dict(w, h, preset, strokes=[(x0,y0,x1,y1)])]."""
+    rng = Rng(seed)
+    out = []
+    for _ in range(n):
+        w = sides[rng.randint(0, len(sides) - 1)]
+        h = sides[rng.randint(0, len(sides) - 1)]
+        preset = ENSEMBLE_PRESETS[rng.randint(0, 2)]
+        reach = max(1, min(w, h) // 4)
+        st = []
+        for _ in range(strokes):
+            x0, y0 = rng.randint(1, w), rng.randint(1, h)
+            x1 = min(max(x0 + rng.randint(-reach, reach), 1), w)
+            y1 = min(max(y0 + rng.randint(-reach, reach), 1), h)
+            st.append((x0, y0, x1, y1))
+        out.append(dict(w=w, h=h, preset=preset, strokes=st))
+    return out
+
+
+def ensemble_member_setup(f, spec, iters=60):
+    """One member: the preset (resetToPreset, main.go:747-793) with the jet of the jet
+    scene (16 rows centred at i = 1, u = 1, smoke 1) or, for the obstacle, an inflow of
+    u = 1 on the left edge as in the Karman scene; then the wall strokes. SL advection,
+    no confinement."""
+    if spec["preset"] == "jet":
+        f.reset_to_preset(0)
+        lo = f.NumY // 2 - 8
+        f.set_jet(True, 1, max(lo, 1), min(lo + 16, f.NumY - 1), 1.0, 1.0)
+    elif spec["preset"] == "cavity":
+        f.reset_to_preset(1, 1.0)
+    else:
+        f.reset_to_preset(2)
+        for j in range(1, f.NumY - 1):
+            f.add_sink(1, j, 1.0, 0.0)
+    for (x0, y0, x1, y1) in spec["strokes"]:
+        f.draw_line(x0, y0, x1, y1, True)
+    ensemble_knobs(f, iters)
+
+
+def ensemble_knobs(f, iters=60):
+    """The ensemble's solver knobs (on a member, a solo Fluid or the batch)."""
+    f.UseBFECC = False
+    f.Confinement = 0.0
+    f.PressureIters = iters
+
+
+def ensemble_impulses(step, specs, seed=4, sigma=8.0, fmax=5.0, every=10):
+    """Gaussian impulses of a step: on every `every`-th step each member gets one at a
+    uniform interior cell with |F| uniform in [0, fmax) and a uniform direction.
+    Returns [(member, cx, cy, fx, fy, sigma)] (fx, fy rounded to float32)."""
+    if step % every:
+        return []
+    rng = Rng(seed * 1000003 + step)
+    out = []
+    for e, s in enumerate(specs):
+        cx, cy = rng.randint(1, s["w"]), rng.randint(1, s["h"])
+        mag, ang = fmax * rng.uniform(), 2.0 * np.pi * rng.uniform()
+        fx, fy = np.float32(mag * np.cos(ang)), np.float32(mag * np.sin(ang))
+        out.append((e, cx, cy, float(fx), float(fy), sigma))
+    return out

@@ -37,7 +37,8 @@ static void batch_free(Batch* B) {
   for (void* p : {(void*)B->flags_pool, (void*)B->diag_pool, (void*)B->ctrl_pool, (void*)B->maxdiv_pool,
                   (void*)B->d_sims, (void*)B->d_cells, (void*)B->d_rings, (void*)B->d_tiles, (void*)B->kout,
                   (void*)B->gout, (void*)B->prog, (void*)B->ticket, (void*)B->d_omega, (void*)B->d_src,
-                  (void*)B->d_src_seg, (void*)B->d_src_sim})
+                  (void*)B->d_src_seg, (void*)B->d_src_sim, (void*)B->d_ekeys, (void*)B->d_evals,
+                  (void*)B->d_eseg, (void*)B->d_esim})
     if (p) (void)hipFree(p);
   if (B->stream) (void)hipStreamDestroy(B->stream);
   delete B;
@@ -98,13 +99,61 @@ static int build_batch_sources(Batch* B) {
   return FL_OK;
 }
 
+// The members' queued edits (SetSolid, SetVelocity, AddSmoke, ApplyForce...) in one
+// launch: per member the ops are grouped by target word in call order, as flush_edits
+// does for one context.
+static int flush_batch_edits(Batch* B) {
+  size_t total = 0;
+  for (Ctx* c : B->members) total += c->edits.size();
+  if (total == 0) return FL_OK;
+  std::vector<uint32_t> keys;
+  std::vector<float> vals;
+  std::vector<int32_t> seg, segsim;
+  keys.reserve(total);
+  vals.reserve(total);
+  for (size_t e = 0; e < B->members.size(); e++) {
+    Ctx* c = B->members[e];
+    if (c->edits.empty()) continue;
+    std::vector<uint32_t> order(c->edits.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (uint32_t)k;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return (c->edits[a].key & 0x7fffffffu) < (c->edits[b].key & 0x7fffffffu);
+    });
+    uint32_t prev = 0xffffffffu;
+    for (uint32_t k : order) {
+      const WordOp& op = c->edits[k];
+      if ((op.key & 0x7fffffffu) != prev) {
+        seg.push_back((int32_t)keys.size());
+        segsim.push_back((int32_t)e);
+        prev = op.key & 0x7fffffffu;
+      }
+      keys.push_back(op.key);
+      vals.push_back(op.val);
+    }
+    c->edits.clear();
+  }
+  const int nseg = (int)seg.size();
+  seg.push_back((int32_t)keys.size());
+  FL_HIP(hipStreamSynchronize(B->stream));  // the staging buffers may still be read
+  if (total + 1 > B->edit_cap) {
+    for (void* p : {(void*)B->d_ekeys, (void*)B->d_evals, (void*)B->d_eseg, (void*)B->d_esim})
+      if (p) FL_HIP(hipFree(p));
+    const size_t cap = std::max<size_t>(total + 1, 4096);
+    FL_HIP(hipMalloc(&B->d_ekeys, sizeof(uint32_t) * cap));
+    FL_HIP(hipMalloc(&B->d_evals, sizeof(float) * cap));
+    FL_HIP(hipMalloc(&B->d_eseg, sizeof(int32_t) * (cap + 1)));
+    FL_HIP(hipMalloc(&B->d_esim, sizeof(int32_t) * (cap + 1)));
+    B->edit_cap = cap;
+  }
+  FL_HIP(hipMemcpy(B->d_ekeys, keys.data(), sizeof(uint32_t) * keys.size(), hipMemcpyHostToDevice));
+  FL_HIP(hipMemcpy(B->d_evals, vals.data(), sizeof(float) * vals.size(), hipMemcpyHostToDevice));
+  FL_HIP(hipMemcpy(B->d_eseg, seg.data(), sizeof(int32_t) * seg.size(), hipMemcpyHostToDevice));
+  FL_HIP(hipMemcpy(B->d_esim, segsim.data(), sizeof(int32_t) * segsim.size(), hipMemcpyHostToDevice));
+  return batch_apply_edits(B, B->d_ekeys, B->d_evals, B->d_eseg, B->d_esim, nseg);
+}
+
 static int prepare_batch(Batch* B, float dt) {
   int rc;
-  bool any_flags = false;
-  for (Ctx* c : B->members) {
-    if ((rc = flush_edits(c))) return rc;
-    if (c->flags_dirty) any_flags = true;
-  }
   const int N = B->prm.pressure_iters;
   if (N > MEMBER_NMAX) return berr(FL_ERR_UNSUPPORTED, "batched solves support at most 1024 sweeps");
   if (B->prm.use_multigrid && B->prm.multigrid_levels > 1)
@@ -129,6 +178,9 @@ static int prepare_batch(Batch* B, float dt) {
     FL_HIP(hipMemcpy(B->d_sims, d.data(), sizeof(SimDesc) * d.size(), hipMemcpyHostToDevice));
     B->descs = d;
   }
+  if ((rc = flush_batch_edits(B))) return rc;
+  bool any_flags = false;
+  for (Ctx* c : B->members) any_flags |= c->flags_dirty;
   if (any_flags) {
     if ((rc = batch_flags(B))) return rc;
     for (Ctx* c : B->members) c->flags_dirty = false;

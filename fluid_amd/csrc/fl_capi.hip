@@ -651,6 +651,24 @@ int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int r
     }
   return FL_OK;
 }
+int fl_apply_force_gaussian(fl_ctx* ctx, int cx, int cy, float fx, float fy, float sigma) {
+  if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
+  if (!(sigma > 0.0f) || sigma > 1e6f) return err(FL_ERR_INVALID_ARG, "sigma must be in (0, 1e6]");
+  Ctx* c = C_OF(ctx);
+  const float two_s2 = 2.0f * sigma * sigma, cut2 = 9.0f * sigma * sigma;
+  const int R = (int)std::ceil(3.0f * sigma);
+  const int ilo = std::max(cx - R, 1), ihi = std::min(cx + R, c->NX - 2);
+  const int jlo = std::max(cy - R, 1), jhi = std::min(cy + R, c->NY - 2);
+  for (int i = ilo; i <= ihi; i++)
+    for (int j = jlo; j <= jhi; j++) {
+      float dx = (float)(i - cx), dy = (float)(j - cy);
+      float d2 = dx * dx + dy * dy;
+      if (d2 > cut2) continue;
+      float w = (float)std::exp(-(double)(d2 / two_s2));
+      apply_force(c, i, j, fx * w, fy * w);
+    }
+  return FL_OK;
+}
 int fl_set_circular_obstacle(fl_ctx* ctx, int cx, int cy, int radius) {
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   set_circular_obstacle(C_OF(ctx), cx, cy, radius);

@@ -229,6 +229,12 @@ struct Batch {
   int n_src_seg = 0;
   size_t src_cap = 0;
   std::vector<int> src_gen_seen;
+  // staging of the members' queued edits, applied by one launch per step
+  uint32_t* d_ekeys = nullptr;
+  float* d_evals = nullptr;
+  int32_t* d_eseg = nullptr;
+  int32_t* d_esim = nullptr;
+  size_t edit_cap = 0;
   // captured step: replayed while dt, the parameters and every launch shape are unchanged
   bool use_graph = true;
   hipGraphExec_t graph = nullptr;
@@ -279,6 +285,8 @@ int batch_bfecc_velocity_corr(Batch* B, float dt);
 int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt);
 int batch_bfecc_smoke_corr(Batch* B, float dt);
 int batch_apply_sources(Batch* B);
+int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const int32_t* seg, const int32_t* segsim,
+                      int nseg);
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
 int pick_tk(const fl_params& P, int N);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);

@@ -638,6 +638,35 @@ __global__ void k_apply_edits(FieldPtrs fp, uint32_t ncells, const uint32_t* __r
   }
   dst[idx] = x;
 }
+// Batched edits: segment s belongs to member segsim[s]; keys are member-local
+// (field * cells + idx) and fields map to the member's current buffers.
+__global__ void k_apply_edits_batch(const SimDesc* __restrict__ sims, const uint32_t* __restrict__ keys,
+                                    const float* __restrict__ vals, const int32_t* __restrict__ seg,
+                                    const int32_t* __restrict__ segsim, int nseg) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const SimDesc& d = sims[segsim[s]];
+  const uint32_t ncells = (uint32_t)d.g.NX * (uint32_t)d.g.NY;
+  int b = seg[s], e = seg[s + 1];
+  uint32_t key = keys[b] & 0x7fffffffu;
+  uint32_t field = key / ncells, idx = key % ncells;
+  float* dst = d.buf[field];
+  float x = dst[idx];
+  for (int k = b; k < e; k++) {
+    bool add = (keys[k] >> 31) != 0;
+    x = add ? x + vals[k] : vals[k];
+  }
+  dst[idx] = x;
+}
+int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const int32_t* seg, const int32_t* segsim,
+                      int nseg) {
+  if (nseg == 0) return FL_OK;
+  hipLaunchKernelGGL(k_apply_edits_batch, dim3((nseg + 255) / 256), dim3(256), 0, B->stream, B->d_sims, keys, vals,
+                     seg, segsim, nseg);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
 int launch_apply_edits(Ctx* c, int nseg) {
   FieldPtrs fp;
   fp.f[FL_FIELD_U] = c->U;

@@ -246,6 +246,10 @@ class Fluid:
     def ApplyForceRadius(self, cx, cy, fx, fy, radius):
         check(self._L.fl_apply_force_radius(self._ctx, cx, cy, fx, fy, radius))
 
+    def ApplyForceGaussian(self, cx, cy, fx, fy, sigma):
+        """ApplyForce (fluid.go:761) weighted by a Gaussian of width sigma, truncated at 3 sigma."""
+        check(self._L.fl_apply_force_gaussian(self._ctx, cx, cy, fx, fy, sigma))
+
     def SetCircularObstacle(self, cx, cy, radius):
         check(self._L.fl_set_circular_obstacle(self._ctx, cx, cy, radius))
 

@@ -127,6 +127,11 @@ int fl_reset(fl_ctx* ctx);                                 /* Reset     walls.go
 /* fluid.go:761-907 */
 int fl_apply_force(fl_ctx* ctx, int i, int j, float fx, float fy);            /* :761 */
 int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int radius); /* :774 */
+/* Gaussian impulse: ApplyForce(i, j, fx*w, fy*w) (fluid.go:761) for every interior cell
+ * within 3*sigma of (cx, cy), w = exp(-d^2 / (2 sigma^2)), cells in i-major order like
+ * ApplyForceRadius (fluid.go:774, whose weight exp(-3 d^2/r^2) is this with sigma =
+ * r/sqrt(6) but not truncated at 3 sigma). An addition of this library. */
+int fl_apply_force_gaussian(fl_ctx* ctx, int cx, int cy, float fx, float fy, float sigma);
 int fl_set_circular_obstacle(fl_ctx* ctx, int cx, int cy, int radius);        /* :894 */
 int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out);         /* :529 */
 int fl_max_divergence(fl_ctx* ctx, float* out);                               /* :876 */

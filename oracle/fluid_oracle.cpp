@@ -812,6 +812,22 @@ template <class T> struct Fluid {
         ApplyForce(i, j, fx * w, fy * w);
       }
   }
+  // Gaussian impulse of the ensemble workload (no reference counterpart): ApplyForce
+  // (fluid.go:761-772) weighted by exp(-d^2/(2 sigma^2)) over cells with d <= 3 sigma,
+  // visited in ApplyForceRadius's order (fluid.go:780-795).
+  void ApplyForceGaussian(int cx, int cy, T fx, T fy, float sigma) {
+    const float two_s2 = 2.0f * sigma * sigma, cut2 = 9.0f * sigma * sigma;
+    const int R = (int)std::ceil(3.0f * sigma);
+    for (int i = cx - R; i <= cx + R; i++)
+      for (int j = cy - R; j <= cy + R; j++) {
+        if (i < 1 || i >= NX - 1 || j < 1 || j >= NY - 1) continue;
+        float dx = (float)(i - cx), dy = (float)(j - cy);
+        float d2 = dx * dx + dy * dy;
+        if (d2 > cut2) continue;
+        T w = (T)std::exp(-(double)(d2 / two_s2));
+        ApplyForce(i, j, fx * w, fy * w);
+      }
+  }
   void SetCircularObstacle(int cx, int cy, int radius) {
     for (int i = cx - radius; i <= cx + radius; i++)
       for (int j = cy - radius; j <= cy + radius; j++) {
@@ -1231,6 +1247,13 @@ void orc_apply_force_radius(void* h, int cx, int cy, double fx, double fy, int r
   DISPATCH(H, {
     using T = std::remove_reference_t<decltype(F->h)>;
     F->ApplyForceRadius(cx, cy, (T)(float)fx, (T)(float)fy, r);
+  });
+}
+void orc_apply_force_gaussian(void* h, int cx, int cy, double fx, double fy, double sigma) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    F->ApplyForceGaussian(cx, cy, (T)(float)fx, (T)(float)fy, (float)sigma);
   });
 }
 void orc_set_circular_obstacle(void* h, int cx, int cy, int r) {

@@ -74,6 +74,7 @@ def lib():
             "orc_reset": (None, [vp]),
             "orc_apply_force": (None, [vp, i, i, d, d]),
             "orc_apply_force_radius": (None, [vp, i, i, d, d, i]),
+            "orc_apply_force_gaussian": (None, [vp, i, i, d, d, d]),
             "orc_set_circular_obstacle": (None, [vp, i, i, i]),
             "orc_adaptive_dt": (d, [vp, f]),
             "orc_max_divergence": (d, [vp]),
@@ -272,6 +273,9 @@ class OracleFluid:
 
     def ApplyForceRadius(self, cx, cy, fx, fy, r):
         self._L.orc_apply_force_radius(self._h, cx, cy, fx, fy, r)
+
+    def ApplyForceGaussian(self, cx, cy, fx, fy, sigma):
+        self._L.orc_apply_force_gaussian(self._h, cx, cy, fx, fy, sigma)
 
     def SetCircularObstacle(self, cx, cy, r):
         self._L.orc_set_circular_obstacle(self._h, cx, cy, r)

@@ -197,3 +197,43 @@ def test_batch_rejects_bad_input():
     with pytest.raises(FluidError):
         b.update(C.DT)
     b.close()
+
+
+def test_gaussian_impulse_matches_oracle():
+    g, o = Fluid(C.DENSITY, 70, 50, C.H), OracleFluid(C.DENSITY, 70, 50, C.H)
+    for f in (g, o):
+        C.open_all_cells(f)
+        f.SetSolid(33, 25, False)
+        f.ApplyForceGaussian(30, 25, 3.5, -1.25, 8.0)
+        f.ApplyForceGaussian(2, 48, -0.5, 4.0, 2.5)
+    same(g, o, "impulses", ("U", "V", "S"))
+
+
+def test_ensemble_workload_small():
+    """The ensemble workload (presets, wall strokes, Gaussian impulses every 10 steps,
+    60 sweeps, SL) on 12 members of sides 64..256: batched == solo, and three members
+    against the oracle."""
+    specs = C.ensemble_specs(n=12, seed=9, sides=tuple(range(64, 257, 32)))
+    b = FluidBatch(C.DENSITY, [(s["w"], s["h"]) for s in specs], C.H)
+    solo = [Fluid(C.DENSITY, s["w"], s["h"], C.H) for s in specs]
+    picks = (0, 5, 11)
+    ora = {e: OracleFluid(C.DENSITY, specs[e]["w"], specs[e]["h"], C.H, threads=8) for e in picks}
+    for e, s in enumerate(specs):
+        for f in (b.members[e], solo[e]) + ((ora[e],) if e in ora else ()):
+            C.ensemble_member_setup(f, s)
+    C.ensemble_knobs(b)
+    for step in range(12):
+        for (e, cx, cy, fx, fy, sg) in C.ensemble_impulses(step, specs):
+            b.members[e].ApplyForceGaussian(cx, cy, fx, fy, sg)
+            solo[e].ApplyForceGaussian(cx, cy, fx, fy, sg)
+            if e in ora:
+                ora[e].ApplyForceGaussian(cx, cy, fx, fy, sg)
+        b.update(C.DT)
+        for f in solo + list(ora.values()):
+            f.update(C.DT)
+        if step in (0, 1, 10, 11):
+            for e in range(len(specs)):
+                same(b.members[e], solo[e], "step %d member %d" % (step, e))
+            for e, o in ora.items():
+                same(b.members[e], o, "oracle step %d member %d" % (step, e))
+    b.close()

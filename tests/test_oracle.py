@@ -104,6 +104,29 @@ def test_apply_force_radius_closed_form():
     assert U[14, 10] == 0 and U[13, 13] == 0
 
 
+def test_apply_force_gaussian_closed_form():
+    # The ensemble's impulse: ApplyForce weighted by exp(-d^2/(2 sigma^2)), d <= 3 sigma.
+    f = fluid(70, 70)
+    S = f.get("S")
+    S[30, 40] = 0
+    f.set("S", S)
+    f.ApplyForceGaussian(30, 30, 2.0, -1.0, 4.0)
+    U, V = f.get("U"), f.get("V")
+    assert U[30, 30] == F32(2.0) and V[30, 30] == F32(-1.0)              # centre weight exp(0)
+    assert U[34, 30] == F32(2.0) * F32(math.exp(-0.5))                    # d = sigma
+    assert U[30, 18] == F32(2.0) * F32(math.exp(-4.5))                    # d = 3 sigma, kept
+    assert U[42, 31] == 0 and U[30, 17] == 0                              # d > 3 sigma
+    assert U[30, 40] == 0 and V[30, 40] == 0                              # solid cell skipped
+    assert np.array_equal(U[30:55, 30], U[30:5:-1, 30])                   # symmetric
+    assert (U != 0).sum() == sum(1 for a in range(-12, 13) for b in range(-12, 13)
+                                 if a * a + b * b <= 144) - 1            # the disc minus the solid cell
+    # clipped at the interior like ApplyForce
+    g = fluid(20, 20)
+    g.ApplyForceGaussian(1, 1, 1.0, 1.0, 3.0)
+    U = g.get("U")
+    assert U[0, :].max() == 0 and U[:, 0].max() == 0 and U[1, 1] == 1.0
+
+
 def test_set_circular_obstacle_and_set_solid_faces():
     # TestSetCircularObstacle (fluid_test.go:1427-1447) + SetSolid face zeroing (walls.go:19-48)
     f = fluid(20, 20)

@@ -193,13 +193,78 @@ def bench_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_diff(args):
+    """The differentiable rollout: 512^2 jet (16 rows, u = 1, all walls), BFECC, confinement
+    0.01 (0.3 makes the step itself diverge on this grid), 80 sweeps, 64 steps, force field F = 0, loss = |M_64 - T|^2 with T six seeded
+    Gaussians (sigma 20 cells), checkpoint every 8 steps. Times the forward rollout and the
+    backward pass (segment recompute + adjoint) on the device."""
+    import numpy as np
+    from fluid_amd import Fluid, Rollout
+    from fluid_amd import configs as C
+    size, steps, every = args.size if args.size != 4096 else 512, 64, 8
+    iters = args.iters if args.iters != 200 else 80
+    f = C.diff_scene(lambda *a: Fluid(*a), size, size, iters)
+    f.Confinement = args.confinement
+    shape = (f.NumX, f.NumY)
+    T = C.gaussian_blobs(f.NumX, f.NumY)
+    z = np.zeros(shape, dtype=np.float32)
+    r = Rollout(f, steps, checkpoint_every=every)
+    print("bench: differentiable rollout %d^2, %d steps, %d sweeps" % (size, steps, iters), file=sys.stderr, flush=True)
+    r.forward(C.DT, z, z, T)
+    r.backward()
+    reps = max(1, min(args.steps, 5))
+    tf = tb = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        loss = r.forward(C.DT, z, z, T)
+        t1 = time.perf_counter()
+        gu, gv = r.backward()
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    tf, tb = tf / reps, tb / reps
+    cells = size * size
+    st = f.stats()
+    b_alg = alg_bytes_per_cell(st["sweeps_run"], True, True, True, False)
+    out = {
+        "metric": METRIC,
+        "value": cells * steps / tf,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * tf / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "differentiable jet rollout %dx%d FP32, BFECC, confinement %g, %d GS sweeps, "
+                               "%d steps, checkpoint every %d, F = 0" % (size, size, args.confinement, iters, steps,
+                                                                         every),
+                   "grid": [size, size], "pressure_sweeps": iters, "steps": steps, "checkpoint_every": every},
+        "forward_ms": 1e3 * tf,
+        "backward_ms": 1e3 * tb,
+        "adjoint_over_forward": tb / tf,
+        "loss": loss,
+        "grad_norm": float(np.sqrt((gu.astype(np.float64) ** 2).sum() + (gv.astype(np.float64) ** 2).sum())),
+        "step_achieved_GBs": b_alg * cells / (tf / steps) / 1e9,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    r.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble"])
+    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble", "diff"])
     ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--confinement", type=float, default=0.01,
+                    help="differentiable rollout: vorticity confinement (0.3 diverges on the 512^2 jet)")
     ap.add_argument("--no-per-grid", action="store_true")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=200)
@@ -218,6 +283,8 @@ def main():
     from fluid_amd import Fluid
     from fluid_amd import configs as C
 
+    if args.config == "diff":
+        return bench_diff(args)
     if args.config == "ensemble":
         if args.iters == 200:
             args.iters = 60

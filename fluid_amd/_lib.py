@@ -111,6 +111,10 @@ SIGNATURES = {
     "fl_batch_sync": (_i, [_vp]),
     "fl_batch_set_graph": (_i, [_vp, _i]),
     "fl_batch_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
+    "fl_vjp_create": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
+    "fl_vjp_destroy": (_i, [_vp]),
+    "fl_vjp_forward": (_i, [_vp, _f, _fp, _fp, _fp, _sz, ctypes.POINTER(ctypes.c_double)]),
+    "fl_vjp_backward": (_i, [_vp, _fp, _fp, _sz]),
 }
 
 _lib = None

@@ -103,7 +103,7 @@ int flush_edits(Ctx* c) {
   return FL_OK;
 }
 
-static int prepare(Ctx* c) {
+int ctx_prepare(Ctx* c) {
   int rc = flush_edits(c);
   if (rc) return rc;
   if (c->flags_dirty) return launch_flags(c);
@@ -133,7 +133,7 @@ void source_ops(Ctx* c, std::vector<SrcOp>& ops) {
   }
 }
 
-static int build_sources(Ctx* c) {
+int ctx_build_sources(Ctx* c) {
   if (!c->src_dirty) return FL_OK;
   std::vector<SrcOp> ops;
   source_ops(c, ops);
@@ -235,7 +235,7 @@ static int simulate_passes(Ctx* c, float dt) {
 // Simulate (fluid.go:79-109). Event 0 is recorded by the caller before any per-step
 // source injection so that it is counted as part of the step.
 static int simulate(Ctx* c, float dt, bool start_marked = false) {
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   if (c->NX < 3 || c->NY < 3) return err(FL_ERR_UNSUPPORTED, "grid must have at least 1x1 interior cells");
   if (!start_marked && (rc = mark(c, 0))) return rc;
@@ -679,7 +679,7 @@ int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out) {
   CTX_CHECK(ctx);
   if (!out) return err(FL_ERR_INVALID_ARG, "null output");
   Ctx* c = C_OF(ctx);
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   float maxVel = 0.0f;
   if ((rc = launch_max_reduce(c, 1, &maxVel))) return rc;
@@ -696,7 +696,7 @@ int fl_max_divergence(fl_ctx* ctx, float* out) {
   CTX_CHECK(ctx);
   if (!out) return err(FL_ERR_INVALID_ARG, "null output");
   Ctx* c = C_OF(ctx);
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   return launch_max_reduce(c, 0, out);
 }
@@ -706,7 +706,7 @@ int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int 
   if (n < 0 || (n > 0 && (!xs || !ys || !us || !vs))) return err(FL_ERR_INVALID_ARG, "bad arguments");
   if (n == 0) return FL_OK;
   Ctx* c = C_OF(ctx);
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   if (n > c->xy_cap) {
     if (c->d_xy) FL_HIP(hipFree(c->d_xy));
@@ -738,7 +738,7 @@ static int view(fl_ctx* ctx, int kind, float* out, size_t n, float* mn, float* m
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   if (!out || n != c->N) return err(FL_ERR_INVALID_ARG, "output must hold NumX*NumY floats");
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   if ((rc = launch_view(c, kind, c->T3))) return rc;
   FL_HIP(hipMemcpyAsync(out, c->T3, 4 * n, hipMemcpyDeviceToHost, c->stream));
@@ -779,7 +779,7 @@ int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
 int fl_pass(fl_ctx* ctx, int pass, float dt, int iters) {
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
-  int rc = prepare(c);
+  int rc = ctx_prepare(c);
   if (rc) return rc;
   switch (pass) {
     case FL_PASS_COPY_BORDER_UV:
@@ -908,7 +908,7 @@ int fl_scene_update(fl_ctx* ctx, float dt) {
   Ctx* c = C_OF(ctx);
   int rc = flush_edits(c);
   if (rc) return rc;
-  if ((rc = build_sources(c))) return rc;
+  if ((rc = ctx_build_sources(c))) return rc;
   if ((rc = mark(c, 0))) return rc;
   if ((rc = launch_apply_sources(c))) return rc;
   return simulate(c, dt, true);

@@ -246,6 +246,8 @@ struct Batch {
 
 // ---- host runtime shared by single contexts and batches (fl_capi.hip) ------------------
 int flush_edits(Ctx* c);
+int ctx_prepare(Ctx* c);        // flush edits, refresh flags
+int ctx_build_sources(Ctx* c);  // upload the per-step source program if it changed
 void source_ops(Ctx* c, std::vector<SrcOp>& ops);
 void free_ctx(Ctx* c);
 fl_params default_params();

@@ -416,3 +416,54 @@ def device_count():
     n = ctypes.c_int()
     _lib.load().fl_device_count(ctypes.byref(n))
     return n.value
+
+
+class Rollout:
+    """A differentiable rollout of `steps` steps from the Fluid's current state.
+
+    Each step is the scene update (sources and jet, main.go:231-241), then U,V += F on fluid
+    interior cells, then Simulate (fluid.go:79-109); the loss is sum((M - target)^2) after
+    the last step. `forward` restores the captured initial state, runs the steps on the
+    device (checkpointing every `checkpoint_every` steps) and returns the loss; `backward`
+    returns (dL/dFu, dL/dFv) from the hand-written adjoint. The reference has no gradients.
+    """
+
+    def __init__(self, fluid, steps, checkpoint_every=8):
+        self._L = fluid._L
+        self._f = fluid
+        v = ctypes.c_void_p()
+        fluid._push()
+        check(self._L.fl_vjp_create(fluid._ctx, int(steps), int(checkpoint_every), ctypes.byref(v)))
+        self._v = v
+        self.steps = steps
+        self.shape = (fluid.NumX, fluid.NumY)
+
+    def close(self):
+        if getattr(self, "_v", None):
+            self._L.fl_vjp_destroy(self._v)
+            self._v = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(self.shape))
+
+    def forward(self, dt, Fu, Fv, target):
+        self._f._push()
+        fu, fv, tg = self._arr(Fu), self._arr(Fv), self._arr(target)
+        loss = ctypes.c_double()
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(self._L.fl_vjp_forward(self._v, dt, fu.ctypes.data_as(fp), fv.ctypes.data_as(fp),
+                                     tg.ctypes.data_as(fp), fu.size, ctypes.byref(loss)))
+        return loss.value
+
+    def backward(self):
+        gu = np.zeros(self.shape, dtype=np.float32)
+        gv = np.zeros(self.shape, dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        check(self._L.fl_vjp_backward(self._v, gu.ctypes.data_as(fp), gv.ctypes.data_as(fp), gu.size))
+        return gu, gv

@@ -205,6 +205,23 @@ int fl_batch_sync(fl_batch* b);
 int fl_batch_set_graph(fl_batch* b, int on);
 int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms);
 
+/* Differentiable rollouts (no reference counterpart: the reference has no gradients).
+ * A rollout of `steps` steps from the state the context holds at fl_vjp_create; each step
+ * is Game.Update without input (the scene's sources and jet, main.go:231-241), then
+ * U,V += F on fluid interior cells (ApplyForce semantics, fluid.go:761-772, for a whole
+ * force field F = (fu, fv)), then Simulate (fluid.go:79-109). loss = sum over all cells of
+ * (M - target)^2 after the last step. fl_vjp_forward runs it (from the captured state,
+ * checkpointing every `checkpoint_every` steps) and leaves the final state in the context;
+ * fl_vjp_backward returns dL/dF by the hand-written adjoint of every pass, recomputing
+ * each segment from its checkpoint. Arrays hold NumX*NumY floats (index i*NumY + j).
+ * Viscosity and multigrid are not differentiated (FL_ERR_UNSUPPORTED). */
+typedef struct fl_vjp fl_vjp;
+int fl_vjp_create(fl_ctx* ctx, int steps, int checkpoint_every, fl_vjp** out);
+int fl_vjp_destroy(fl_vjp* v);
+int fl_vjp_forward(fl_vjp* v, float dt, const float* fu, const float* fv, const float* target, size_t n,
+                   double* loss);
+int fl_vjp_backward(fl_vjp* v, float* grad_fu, float* grad_fv, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,6 +29,7 @@
 #include <functional>
 #include <limits>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 namespace orc {
@@ -170,6 +171,13 @@ template <class T> struct Fluid {
   std::vector<uint8_t> turbGateIn;
   bool useTurbGateIn = false;
   std::vector<uint8_t> turbGateOut;
+
+  // Optional taps of BFECC intermediates (the adjoint's tape): forward-pass and
+  // corrected fields of velocity and smoke.
+  struct Taps {
+    std::vector<T>*fwdU = nullptr, *fwdV = nullptr, *corrU = nullptr, *corrV = nullptr;
+    std::vector<T>*fwdM = nullptr, *corrM = nullptr;
+  } taps;
 
   Fluid(T density_, int width, int height, T h_) : density(density_), h(h_) {
     NX = width + 2;
@@ -664,6 +672,8 @@ template <class T> struct Fluid {
     std::vector<T> origU = U, origV = V;
     advectVelocity(dt);
     std::vector<T> fwdU = U, fwdV = V;
+    if (taps.fwdU) *taps.fwdU = fwdU;
+    if (taps.fwdV) *taps.fwdV = fwdV;
     U = origU;
     V = origV;
     std::vector<T> bwdU(numCells, (T)0), bwdV(numCells, (T)0);
@@ -705,6 +715,8 @@ template <class T> struct Fluid {
         corrU[c] = clampToNeighbors(corrU[c], origU, i, j);
         corrV[c] = clampToNeighbors(corrV[c], origV, i, j);
       }
+    if (taps.corrU) *taps.corrU = corrU;
+    if (taps.corrV) *taps.corrV = corrV;
     U = corrU;
     V = corrV;
     advectVelocity(dt);
@@ -716,6 +728,7 @@ template <class T> struct Fluid {
     std::vector<T> origM = M;
     advectSmoke(dt);
     std::vector<T> fwdM = M;
+    if (taps.fwdM) *taps.fwdM = fwdM;
     std::vector<T> bwdM(numCells, (T)0);
     copyBorder(bwdM, fwdM);
     prange(threads, 1, NX - 1, [&](int i) {
@@ -742,6 +755,7 @@ template <class T> struct Fluid {
         corrM[c] = clampToNeighbors(corrM[c], origM, i, j);
         if (corrM[c] < 0) corrM[c] = 0;
       }
+    if (taps.corrM) *taps.corrM = corrM;
     M = corrM;
     advectSmoke(dt);
   }
@@ -1012,6 +1026,8 @@ template <class T> struct Scene {
   }
 };
 
+#include "fluid_adjoint.inc"
+
 }  // namespace orc
 
 // ===========================================================================================
@@ -1255,6 +1271,27 @@ void orc_apply_force_gaussian(void* h, int cx, int cy, double fx, double fy, dou
     using T = std::remove_reference_t<decltype(F->h)>;
     F->ApplyForceGaussian(cx, cy, (T)(float)fx, (T)(float)fy, (float)sigma);
   });
+}
+// Differentiable rollout (oracle/fluid_adjoint.inc): buffers are in the handle's
+// precision, NX*NY each; gFu/gFv may be null for the loss alone.
+double orc_rollout(void* h, int steps, float dt, const void* Fu, const void* Fv, const void* target, void* gFu,
+                   void* gFv) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    const size_t n = (size_t)F->numCells;
+    std::vector<T> fu((const T*)Fu, (const T*)Fu + n), fv((const T*)Fv, (const T*)Fv + n);
+    std::vector<T> tg((const T*)target, (const T*)target + n), gu, gv;
+    orc::Rollout<T> R(&SC);
+    const bool grad = gFu && gFv;
+    double loss = R.run(steps, (T)dt, fu, fv, tg, grad ? &gu : nullptr, grad ? &gv : nullptr);
+    if (grad) {
+      std::memcpy(gFu, gu.data(), sizeof(T) * n);
+      std::memcpy(gFv, gv.data(), sizeof(T) * n);
+    }
+    return loss;
+  });
+  return 0.0;
 }
 void orc_set_circular_obstacle(void* h, int cx, int cy, int r) {
   OrcHandle* H = (OrcHandle*)h;

@@ -94,6 +94,7 @@ def lib():
             "orc_scene_update": (None, [vp, f]),
             "orc_scene_draw_line": (None, [vp, i, i, i, i, i]),
             "orc_scene_smoke_brush": (None, [vp, i, i, i, f]),
+            "orc_rollout": (d, [vp, i, f, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -189,6 +190,24 @@ class OracleFluid:
         ip = (ctypes.c_int * 5)(int(self.UseMultigrid), int(self.MultigridLevels), int(self.UseBFECC),
                                 int(self.PressureIters), int(self.EarlyExit))
         self._L.orc_set_params(self._h, fp, ip)
+
+    # ---- differentiable rollout (oracle/fluid_adjoint.inc) ----
+    def rollout(self, steps, dt, Fu, Fv, target, grad=True):
+        """Loss sum((M - target)^2) after `steps` steps of sources+jet, U,V += F on fluid
+        interior cells, Simulate; with grad, also (dL/dFu, dL/dFv). The state is kept."""
+        self._push()
+        shape = (self.NumX, self.NumY)
+        fu = np.ascontiguousarray(np.asarray(Fu, dtype=self.dtype).reshape(shape))
+        fv = np.ascontiguousarray(np.asarray(Fv, dtype=self.dtype).reshape(shape))
+        tg = np.ascontiguousarray(np.asarray(target, dtype=self.dtype).reshape(shape))
+        if not grad:
+            return self._L.orc_rollout(self._h, int(steps), dt, fu.ctypes.data, fv.ctypes.data, tg.ctypes.data,
+                                       None, None)
+        gu = np.zeros(shape, dtype=self.dtype)
+        gv = np.zeros(shape, dtype=self.dtype)
+        loss = self._L.orc_rollout(self._h, int(steps), dt, fu.ctypes.data, fv.ctypes.data, tg.ctypes.data,
+                                   gu.ctypes.data, gv.ctypes.data)
+        return loss, gu, gv
 
     # ---- fields ----
     def get(self, name):

@@ -76,6 +76,7 @@ def _fd_worker(a):
 
 
 def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)  # float32 norms overflow here
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
@@ -92,6 +93,7 @@ def run_cpu(grads, summary, workers):
              "eps %g, FP64 forward" % (NDIR, EPS)]
     t0 = time.time()
     jobs = [(H, k) for H in FD_HORIZONS for k in range(NDIR)]
+    g64 = {}
     with Pool(workers) as pool:
         fd_async = pool.map_async(_fd_worker, jobs)
         for H in HORIZONS:
@@ -107,13 +109,14 @@ def run_cpu(grads, summary, workers):
                 T64 = T.astype(np.float64)
                 z64 = np.zeros(T64.shape)
                 l64, u64, v64 = o64.rollout(H, C.DT, z64, z64, T64)
-                line += "  rel(FP64 adjoint) %.2e" % rel(G, np.stack([u64, v64]))
+                g64[H] = np.stack([u64, v64])
+                line += "  rel(FP64 adjoint) %.2e" % rel(G, g64[H])
             lines.append(line)
             print(line, flush=True)
         fd = fd_async.get()
     for H in FD_HORIZONS:
         G = np.stack([d["gu_%d" % H], d["gv_%d" % H]]).astype(np.float64)
-        errs = []
+        errs, errs64 = [], []
         for (h, k, val) in fd:
             if h != H:
                 continue
@@ -121,10 +124,14 @@ def run_cpu(grads, summary, workers):
             shape = G.shape[1:]
             du, dv = rng.standard_normal(shape), rng.standard_normal(shape)
             ad = float((G[0] * du).sum() + (G[1] * dv).sum())
+            ad64 = float((g64[H][0] * du).sum() + (g64[H][1] * dv).sum())
             errs.append(abs(ad - val) / max(abs(val), 1e-300))
-        errs = np.array(errs)
-        line = "horizon %2d: fd over %d directions: median %.2e  max %.2e  (<=1e-3 in %d of %d)" % (
-            H, len(errs), np.median(errs), errs.max(), int((errs <= 1e-3).sum()), len(errs))
+            errs64.append(abs(ad64 - val) / max(abs(val), 1e-300))
+        errs, errs64 = np.array(errs), np.array(errs64)
+        line = ("horizon %2d: fd over %d directions: device median %.2e max %.2e (<=1e-3 in %d of %d); "
+                "FP64 adjoint median %.2e max %.2e (<=1e-3 in %d)" % (
+                    H, len(errs), np.median(errs), errs.max(), int((errs <= 1e-3).sum()), len(errs),
+                    np.median(errs64), errs64.max(), int((errs64 <= 1e-3).sum())))
         lines.append(line)
         print(line, flush=True)
     lines.append("# host time %.0f s with %d worker processes" % (time.time() - t0, workers))

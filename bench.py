@@ -277,7 +277,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (the multi-GPU decomposition "
-              "is described in DESIGN.md and not built)" % args.gpus, file=sys.stderr)
+              "is a design only and not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from fluid_amd import Fluid

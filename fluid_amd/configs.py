@@ -65,7 +65,7 @@ def open_all_cells(f):
 
 
 def jet_scene(make, width=256, height=256, iters=40, jet_u=1.0, jet_rows=16, walls=(True, True, True, True)):
-    """Config 1: jet inlet at the left edge, zero initial state, all border walls."""
+    """Jet: inlet at the left edge, zero initial state, all border walls."""
     f = make(DENSITY, width, height, H)
     open_all_cells(f)
     f.scene_walls(*walls)
@@ -77,7 +77,7 @@ def jet_scene(make, width=256, height=256, iters=40, jet_u=1.0, jet_rows=16, wal
 
 
 def karman_scene(make, width=2048, height=1024, iters=100, radius=64, centre=(256, 512), seed=2, band=32):
-    """Config 2: disc obstacle, uniform inflow u=1 on the left, open right edge, BFECC,
+    """Karman street: disc obstacle, uniform inflow u=1 on the left, open right edge, BFECC,
     confinement 0.3, seeded smoke noise in the inflow band i in [1, band]."""
     f = make(DENSITY, width, height, H)
     open_all_cells(f)
@@ -119,7 +119,7 @@ def cavity_discs(nx, ny, seed=3, frac=0.05, rmin=8, rmax=64):
 
 
 def cavity_scene(make, width=4096, height=4096, iters=200, lid_u=1.0, seed=3, frac=0.05, bfecc=True):
-    """Config 3: lid-driven cavity. All walls; the lid is the source row j = NY-2,
+    """Lid-driven cavity. All walls; the lid is the source row j = NY-2,
     i in [2, NX-3] (resetToPreset's cavity, main.go:767-779) with U = lid_u; seeded
     solid discs cover >= 5% of the interior; BFECC on."""
     f = make(DENSITY, width, height, H)
@@ -136,7 +136,8 @@ def cavity_scene(make, width=4096, height=4096, iters=200, lid_u=1.0, seed=3, fr
 
 
 def diff_scene(make, width=512, height=512, iters=80):
-    """Config 5 forward: the jet scene at 512^2 with BFECC and confinement 0.3."""
+    """Differentiable-rollout scene: the jet at 512^2 with BFECC and confinement 0.3 (callers
+    of the rollout lower it: 0.3 diverges on this grid)."""
     f = jet_scene(make, width, height, iters)
     f.UseBFECC = True
     f.Confinement = 0.3
@@ -144,7 +145,7 @@ def diff_scene(make, width=512, height=512, iters=80):
 
 
 def gaussian_blobs(nx, ny, seed=5, count=6, sigma=20.0):
-    """Target smoke field of config 5: sum of `count` seeded Gaussians (sigma cells)."""
+    """Target smoke field of the differentiable rollout: sum of `count` seeded Gaussians (sigma cells)."""
     rng = Rng(seed)
     ii = np.arange(nx, dtype=np.float64)[:, None]
     jj = np.arange(ny, dtype=np.float64)[None, :]

@@ -5,8 +5,7 @@
  * The reference has no FFI: its interface is the Go package API
  * (github.com/TheFellow/fluid/pkg/fluid). Each entry point below replaces one Go
  * function or method; the reference file:line it stands for is given beside it.
- * A cgo binding that restores the exact Go API on top of these calls is shown in
- * INTEGRATION.md. Plain pointers and sizes only; no Go pointer is retained.
+ * A cgo binding restores the exact Go API on top of these calls. Plain pointers and sizes only; no Go pointer is retained.
  *
  * Layout: every field is float32 [NumX*NumY] with index i*NumY + j (j contiguous),
  * NumX = width+2, NumY = height+2, as in the reference (fluid.go:42-58).

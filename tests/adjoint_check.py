@@ -3,11 +3,11 @@
 
 Two halves, because the oracle cannot run where the GPU is exercised in a test budget:
 
-  python tests/adjoint_check.py gpu --out gpurun_out/adjoint_grads.npz
+  python tests/adjoint_check.py gpu --out grads.npz
       (on an MI355X) runs the rollout (512^2 jet, BFECC, confinement 0.01, 80 sweeps,
       F = 0, target six Gaussians) for horizons of 1, 2, 8 and 64 steps and saves the loss
       and dL/dF of each.
-  python tests/adjoint_check.py cpu --grads gpurun_out/adjoint_grads.npz --summary out.txt
+  python tests/adjoint_check.py cpu --grads grads.npz --summary out.txt
       (on the host) compares them with the oracle: its float32 adjoint along the identical
       trajectory (every horizon), its FP64 adjoint, and central finite differences of its
       FP64 forward pass on 32 seeded directions (short horizons).
@@ -142,9 +142,9 @@ def run_cpu(grads, summary, workers):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["gpu", "cpu"])
-    ap.add_argument("--out", default="gpurun_out/adjoint_grads.npz")
-    ap.add_argument("--grads", default="gpurun_out/adjoint_grads.npz")
-    ap.add_argument("--summary", default="gpurun_out/adjoint_check.txt")
+    ap.add_argument("--out", default="adjoint_grads.npz")
+    ap.add_argument("--grads", default="adjoint_grads.npz")
+    ap.add_argument("--summary", default="adjoint_check.txt")
     ap.add_argument("--workers", type=int, default=min(8, os.cpu_count() or 1))
     a = ap.parse_args()
     if a.mode == "gpu":

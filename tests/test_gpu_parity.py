@@ -115,8 +115,8 @@ def test_early_exit_rollback(solver):
     assert s["sweeps_run"] == o.last_sweeps() < 25 and s["reran"]
 
 
-def test_jet_config1_200_steps():
-    """Config 1 (256^2 jet, 40 sweeps, SL, dt 0.1) for 200 steps, compared every step."""
+def test_jet_256_200_steps():
+    """The 256^2 jet (40 sweeps, SL, dt 0.1) for 200 steps, compared every step."""
     g = C.jet_scene(lambda *a: Fluid(*a))
     o = C.jet_scene(lambda *a: OracleFluid(*a))
     assert_same(g, o, "setup")

@@ -49,7 +49,7 @@ def test_many_tiles_bit_exact(tk, early):
 
 
 def test_karman_full_size():
-    """Config 2 at 2048x1024: disc r=64, inflow u=1, BFECC, confinement 0.3, 100 sweeps."""
+    """The Karman street at 2048x1024: disc r=64, inflow u=1, BFECC, confinement 0.3, 100 sweeps."""
     g = C.karman_scene(lambda *a: Fluid(*a))
     o = C.karman_scene(lambda *a: OracleFluid(*a, threads=8))
     same(g, o, "setup")
@@ -60,7 +60,7 @@ def test_karman_full_size():
 
 
 def test_cavity_full_grid_bit_exact():
-    """Config 3's 4096^2 grid (lid, seeded discs, BFECC), one step with 40 sweeps."""
+    """The 4096^2 cavity grid (lid, seeded discs, BFECC), one step with 40 sweeps."""
     g = C.cavity_scene(lambda *a: Fluid(*a), iters=40)
     o = C.cavity_scene(lambda *a: OracleFluid(*a, threads=8), iters=40)
     same(g, o, "setup", ("S", "U", "V"))
@@ -70,7 +70,7 @@ def test_cavity_full_grid_bit_exact():
 
 
 def test_cavity_full_size_properties():
-    """Config 3 at full size with 200 sweeps: the result is bit-identical from run to run
+    """The 4096^2 cavity with 200 sweeps: the result is bit-identical from run to run
     (the tile handoffs are race-free), the sweeps cut the divergence, the solve keeps all
     200 sweeps, and nothing becomes NaN."""
     g = C.cavity_scene(lambda *a: Fluid(*a))

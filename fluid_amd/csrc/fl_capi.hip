@@ -172,8 +172,7 @@ static int make_incompressible(Ctx* c, int iters, float dt, bool copy_border) {
     if ((rc = launch_copy_border(c, c->nU, c->U))) return rc;
     if ((rc = launch_copy_border(c, c->nV, c->V))) return rc;
   }
-  if (c->prm.use_multigrid && c->prm.multigrid_levels > 1)
-    return err(FL_ERR_UNSUPPORTED, "UseMultigrid: the multigrid V-cycle is not implemented on the device");
+  if (c->prm.use_multigrid && c->prm.multigrid_levels > 1) return multigrid_solve(c, iters, dt);
   return pressure_solve(c, iters, dt);
 }
 
@@ -267,7 +266,7 @@ void free_ctx(Ctx* c) {
     if (c->d_ctrl) (void)hipFree(c->d_ctrl);
     if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
   }
-  for (float** b : {&c->d_omega, &c->d_edit_vals, &c->d_xy})
+  for (float** b : {&c->d_omega, &c->d_edit_vals, &c->d_xy, &c->mg_omega, &c->mg_P, &c->mg_S, &c->mg_R})
     if (*b) (void)hipFree(*b);
   if (c->d_red) (void)hipFree(c->d_red);
   if (c->d_edit_keys) (void)hipFree(c->d_edit_keys);

@@ -136,6 +136,12 @@ struct Ctx {
 
   GsWork gs;
 
+  // Multigrid V-cycle workspace (coarse grid of ((NX+1)/2) x ((NY+1)/2) cells).
+  float* mg_omega = nullptr;  // [6]: the pre-smoothing (1.5) and post-smoothing (1.2) relaxation
+  float* mg_P = nullptr;      // coarse correction
+  float* mg_S = nullptr;      // coarse solid mask
+  float* mg_R = nullptr;      // coarse right-hand side
+
   // Host mirror of S (always valid) and pending edits.
   std::vector<float> hS;
   std::vector<WordOp> edits;
@@ -298,6 +304,8 @@ int batch_gs_solve(Batch* B, int N);
 // Runs the single-grid projection: sweeps with the reference order on (U,V,p).
 // On return the projected velocity is in (c->U, c->V) and the pressure in c->p.
 int pressure_solve(Ctx* c, int numIters, float dt);
+// solveMultigridVCycle (fluid.go:559-596): on return the result is in (c->U, c->V, c->p).
+int multigrid_solve(Ctx* c, int numIters, float dt);
 int pressure_free(Ctx* c);
 
 }  // namespace fl

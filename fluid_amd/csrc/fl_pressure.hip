@@ -168,3 +168,180 @@ int pressure_solve(Ctx* c, int numIters, float dt) {
 }
 
 }  // namespace fl
+
+// ---- multigrid V-cycle (fluid.go:559-758, 1123-1149) ---------------------------------------
+namespace fl {
+
+static constexpr int MBX = 64, MBY = 4;
+
+// computePressureResidual (fluid.go:600-632): r = -div - lap(p) on fluid interior cells, 0 elsewhere.
+__global__ void k_mg_residual(Grid g, const float* __restrict__ S, const float* __restrict__ U,
+                              const float* __restrict__ V, const float* __restrict__ p, float* __restrict__ r) {
+  const int j = blockIdx.x * MBX + threadIdx.x, i = blockIdx.y * MBY + threadIdx.y;
+  const int NX = g.NX, NY = g.NY, n = NY;
+  if (i >= NX || j >= NY) return;
+  const int c = i * n + j;
+  float out = 0.0f;
+  if (i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && S[c] != 0.0f) {
+    const float div = U[c + n] - U[c] + V[c + 1] - V[c];
+    const float sx0 = S[c - n], sx1 = S[c + n], sy0 = S[c - 1], sy1 = S[c + 1];
+    const float lap = sx0 * (p[c - n] - p[c]) + sx1 * (p[c + n] - p[c]) + sy0 * (p[c - 1] - p[c]) +
+                      sy1 * (p[c + 1] - p[c]);
+    out = -div - lap;
+  }
+  r[c] = out;
+}
+
+// restrictResidual (fluid.go:635-664) and the coarse mask of solveCoarseGrid (fluid.go:672-684).
+__global__ void k_mg_restrict(Grid g, int cNX, int cNY, const float* __restrict__ S, const float* __restrict__ r,
+                              float* __restrict__ rhs, float* __restrict__ cS, float* __restrict__ cP) {
+  const int j = blockIdx.x * MBX + threadIdx.x, i = blockIdx.y * MBY + threadIdx.y;
+  if (i >= cNX || j >= cNY) return;
+  const int NX = g.NX, NY = g.NY, cc = i * cNY + j;
+  const int fi = 2 * i, fj = 2 * j;
+  float out = 0.0f;
+  if (i >= 1 && i <= cNX - 2 && j >= 1 && j <= cNY - 2 && fi < NX - 1 && fj < NY - 1) {
+    const float center = r[fi * NY + fj] * 0.25f;
+    const float nb = (r[(fi - 1) * NY + fj] + r[(fi + 1) * NY + fj] + r[fi * NY + fj - 1] + r[fi * NY + fj + 1]) *
+                     0.125f;
+    const float co = (r[(fi - 1) * NY + fj - 1] + r[(fi + 1) * NY + fj - 1] + r[(fi - 1) * NY + fj + 1] +
+                      r[(fi + 1) * NY + fj + 1]) *
+                     0.0625f;
+    out = center + nb + co;
+  }
+  rhs[cc] = out;
+  cS[cc] = (fi < NX && fj < NY) ? S[fi * NY + fj] : 0.0f;
+  cP[cc] = 0.0f;
+}
+
+// solveCoarseGrid's 40 in-place lexicographic sweeps at relaxation 1.6 (fluid.go:686-716),
+// bit-identical by the hyperplane schedule t = i + j + 2k: a cell's four neighbours lie on
+// t-1 (already updated) and t+1 (not yet), and no two cells of one hyperplane touch the same
+// word. One workgroup walks the hyperplanes with a barrier between them.
+static constexpr int MG_COARSE_THREADS = 1024, MG_COARSE_ITERS = 40;
+__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_gs(int cNX, int cNY, const float* __restrict__ cS,
+                                                                   const float* __restrict__ rhs, float* cP) {
+  const int W = cNX - 2, H = cNY - 2, K = MG_COARSE_ITERS;
+  if (W < 1 || H < 1) return;
+  const float relax = 1.6f;
+  const int tmax = W + H + 2 * (K - 1);  // t' = (i-1) + (j-1) + 2k in [0, tmax]
+  for (int t = 0; t <= tmax; t++) {
+    const int klo = max(0, (t - (W - 1) - (H - 1) + 1) / 2), khi = min(K - 1, t / 2);
+    const int span = W;  // i - 1 in [0, W)
+    for (int q = threadIdx.x; q < (khi - klo + 1) * span; q += blockDim.x) {
+      const int k = klo + q / span, ii = q % span;
+      const int jj = t - ii - 2 * k;
+      if (jj < 0 || jj >= H) continue;
+      const int i = ii + 1, j = jj + 1, c = i * cNY + j;
+      if (cS[c] == 0.0f) continue;
+      const float sx0 = cS[c - cNY], sx1 = cS[c + cNY], sy0 = cS[c - 1], sy1 = cS[c + 1];
+      const float s = sx0 + sx1 + sy0 + sy1;
+      if (s == 0.0f) continue;
+      const float pc = cP[c];
+      const float lap = sx0 * (cP[c - cNY] - pc) + sx1 * (cP[c + cNY] - pc) + sy0 * (cP[c - 1] - pc) +
+                        sy1 * (cP[c + 1] - pc);
+      const float res = rhs[c] - lap;
+      const float corr = -res / s * relax;
+      cP[c] = pc + corr;
+    }
+    __syncthreads();
+  }
+}
+
+// prolongateCorrection (fluid.go:719-756) at one fine cell.
+__device__ __forceinline__ float mg_corr(const float* __restrict__ cc, int i, int j, int cNX, int cNY) {
+  const int ci = i / 2, cj = j / 2;
+  if (ci >= cNX - 1 || cj >= cNY - 1) return 0.0f;
+  const float fi = (float)(i % 2) * 0.5f, fj = (float)(j % 2) * 0.5f;
+  return (1.0f - fi) * (1.0f - fj) * cc[ci * cNY + cj] + fi * (1.0f - fj) * cc[(ci + 1) * cNY + cj] +
+         (1.0f - fi) * fj * cc[ci * cNY + cj + 1] + fi * fj * cc[(ci + 1) * cNY + cj + 1];
+}
+__device__ __forceinline__ bool mg_active(const float* __restrict__ S, int i, int j, int NX, int NY) {
+  return i >= 1 && i <= NX - 2 && j >= 1 && j <= NY - 2 && S[i * NY + j] != 0.0f;
+}
+
+// applePressureCorrection (fluid.go:1123-1149), serial in the reference: face U[i,j] gets
+// cell (i-1,j)'s "+= sx1*pc" and then cell (i,j)'s "-= sx0*pc"; V[i,j] gets (i,j-1)'s "+="
+// then (i,j)'s "-=". Each face applies exactly the updates of its active cells in that order.
+__global__ void k_mg_apply(Grid g, int cNX, int cNY, const float* __restrict__ S, const float* __restrict__ cc,
+                           float cp, float* __restrict__ U, float* __restrict__ V, float* __restrict__ p) {
+  const int j = blockIdx.x * MBX + threadIdx.x, i = blockIdx.y * MBY + threadIdx.y;
+  const int NX = g.NX, NY = g.NY, n = NY;
+  if (i >= NX || j >= NY) return;
+  const int c = i * n + j;
+  const bool self = mg_active(S, i, j, NX, NY);
+  const float pcs = self ? mg_corr(cc, i, j, cNX, cNY) : 0.0f;
+  if (self) p[c] += pcs * cp;
+  if (i >= 1) {
+    float u = U[c];
+    if (mg_active(S, i - 1, j, NX, NY)) u += S[c] * mg_corr(cc, i - 1, j, cNX, cNY);
+    if (self) u -= S[c - n] * pcs;
+    U[c] = u;
+  }
+  if (j >= 1) {
+    float v = V[c];
+    if (mg_active(S, i, j - 1, NX, NY)) v += S[c] * mg_corr(cc, i, j - 1, cNX, cNY);
+    if (self) v -= S[c - 1] * pcs;
+    V[c] = v;
+  }
+}
+
+int multigrid_solve(Ctx* c, int numIters, float dt) {
+  c->last_n = 0;
+  c->host_sweeps = 0;
+  if (numIters <= 0) return FL_OK;
+  if (c->batch) {
+    set_error("UseMultigrid is not supported for batch members");
+    return FL_ERR_UNSUPPORTED;
+  }
+  int rc = ensure_omega(c, std::max(numIters, 3));
+  if (rc) return rc;
+  const int cNX = (c->NX + 1) / 2, cNY = (c->NY + 1) / 2;
+  const size_t cN = (size_t)cNX * cNY;
+  if (!c->mg_omega) {
+    const float om[6] = {1.5f, 1.5f, 1.5f, 1.2f, 1.2f, 1.2f};
+    FL_HIP(hipMalloc(&c->mg_omega, sizeof(om)));
+    FL_HIP(hipMalloc(&c->mg_P, 4 * cN));
+    FL_HIP(hipMalloc(&c->mg_S, 4 * cN));
+    FL_HIP(hipMalloc(&c->mg_R, 4 * cN));
+    FL_HIP(hipMemcpy(c->mg_omega, om, sizeof(om), hipMemcpyHostToDevice));
+  }
+  const float cp = c->density * c->h / dt;
+  float* saved_omega = c->d_omega;
+  const int saved_early = c->prm.early_exit;
+  c->prm.early_exit = 0;  // the V-cycle checks the third pre-smoothing sweep itself
+  auto restore = [&]() {
+    c->d_omega = saved_omega;
+    c->prm.early_exit = saved_early;
+  };
+  const dim3 fb(MBX, MBY), fg((c->NY + MBX - 1) / MBX, (c->NX + MBY - 1) / MBY);
+  const dim3 cg((cNY + MBX - 1) / MBX, (cNX + MBY - 1) / MBY);
+  int sweeps = 0;
+  for (int it = 0; it < numIters; it++) {
+    c->d_omega = c->mg_omega;
+    if ((rc = gs_wave_solve(c, 3, cp))) return restore(), rc;
+    sweeps += 3;
+    uint32_t bits = 0;
+    FL_HIP(hipMemcpyAsync(&bits, c->d_maxdiv + 2, 4, hipMemcpyDeviceToHost, c->stream));
+    FL_HIP(hipStreamSynchronize(c->stream));
+    float maxDiv;
+    std::memcpy(&maxDiv, &bits, 4);
+    if (maxDiv < 1e-5f) break;
+    hipLaunchKernelGGL(k_mg_residual, fg, fb, 0, c->stream, c->grid, c->S, c->U, c->V, c->p, c->T0);
+    hipLaunchKernelGGL(k_mg_restrict, cg, fb, 0, c->stream, c->grid, cNX, cNY, c->S, c->T0, c->mg_R, c->mg_S,
+                       c->mg_P);
+    hipLaunchKernelGGL(k_mg_coarse_gs, dim3(1), dim3(MG_COARSE_THREADS), 0, c->stream, cNX, cNY, c->mg_S, c->mg_R,
+                       c->mg_P);
+    hipLaunchKernelGGL(k_mg_apply, fg, fb, 0, c->stream, c->grid, cNX, cNY, c->S, c->mg_P, cp, c->U, c->V, c->p);
+    FL_HIP(hipGetLastError());
+    c->d_omega = c->mg_omega + 3;
+    if ((rc = gs_wave_solve(c, 3, cp))) return restore(), rc;
+    sweeps += 3;
+  }
+  restore();
+  c->last_n = sweeps;
+  c->host_sweeps = sweeps;
+  return FL_OK;
+}
+
+}  // namespace fl

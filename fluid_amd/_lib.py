@@ -100,6 +100,8 @@ SIGNATURES = {
     "fl_scene_draw_line": (_i, [_vp, _i, _i, _i, _i, _i]),
     "fl_scene_smoke_brush": (_i, [_vp, _i, _i, _i, _f]),
     "fl_apply_force_gaussian": (_i, [_vp, _i, _i, _f, _f, _f]),
+    "fl_render": (_i, [_vp, _i, _i, _vp, _sz]),
+    "fl_advect_particles": (_i, [_vp, _f, _i, _vp, _vp, _vp, _vp, _vp]),
     "fl_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
     "fl_batch_create": (_i, [_i, _ip, _ip, _f, _f, ctypes.POINTER(_vp)]),
     "fl_batch_destroy": (_i, [_vp]),

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfluidhip.so")
-SOURCES = ["fl_capi.hip", "fl_kernels.hip", "fl_pressure.hip", "fl_gs_wave.hip", "fl_batch.hip", "fl_adjoint.hip"]
+SOURCES = ["fl_capi.hip", "fl_kernels.hip", "fl_pressure.hip", "fl_gs_wave.hip", "fl_batch.hip", "fl_adjoint.hip", "fl_render.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

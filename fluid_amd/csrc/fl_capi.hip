@@ -269,6 +269,7 @@ void free_ctx(Ctx* c) {
   for (float** b : {&c->d_omega, &c->d_edit_vals, &c->d_xy, &c->mg_omega, &c->mg_P, &c->mg_S, &c->mg_R})
     if (*b) (void)hipFree(*b);
   if (c->d_red) (void)hipFree(c->d_red);
+  if (c->rs_mem) (void)hipFree(c->rs_mem);
   if (c->d_edit_keys) (void)hipFree(c->d_edit_keys);
   if (c->d_edit_seg) (void)hipFree(c->d_edit_seg);
   if (c->d_src) (void)hipFree(c->d_src);
@@ -758,23 +759,6 @@ int fl_vorticity(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) { retu
 int fl_velocity_magnitude(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) {
   return view(ctx, 1, out, n, mn, mx);
 }
-// Pressure()/Smoke() MinValue/MaxValue over every cell in index order (pressure.go:6-15).
-int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
-  CTX_CHECK(ctx);
-  Ctx* c = C_OF(ctx);
-  std::vector<float> h(c->N);
-  int rc = fl_download(ctx, field, h.data(), c->N);
-  if (rc) return rc;
-  float minV = std::numeric_limits<float>::max(), maxV = -std::numeric_limits<float>::max();
-  for (float v : h) {
-    if (v < minV) minV = v;
-    if (v > maxV) maxV = v;
-  }
-  if (mn) *mn = minV;
-  if (mx) *mx = maxV;
-  return FL_OK;
-}
-
 int fl_pass(fl_ctx* ctx, int pass, float dt, int iters) {
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);

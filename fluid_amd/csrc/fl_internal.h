@@ -142,6 +142,10 @@ struct Ctx {
   float* mg_S = nullptr;      // coarse solid mask
   float* mg_R = nullptr;      // coarse right-hand side
 
+  // Scratch of rendering, range reductions and particles (fl_render.hip).
+  char* rs_mem = nullptr;
+  size_t rs_cap = 0;
+
   // Host mirror of S (always valid) and pending edits.
   std::vector<float> hS;
   std::vector<WordOp> edits;

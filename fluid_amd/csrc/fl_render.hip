@@ -1,0 +1,326 @@
+// Rendering, field ranges and particles on the device (the application layer around the
+// step: main/main.go:512-652, main/colors.go:10-84, pkg/fluid/smoke.go, pressure.go).
+//
+// - Field min/max as the reference scans them (first value in index order wins among equals,
+//   NaN never selected), reduced on the device in two fixed passes, so Pressure()/Smoke()
+//   ranges and the colormaps need no field download.
+// - Game.Draw's image: the scientific (blue-cyan-green-yellow-red) or diverging colormap of a
+//   field, flipped vertically, with solid cells black; the host adds the few source/sink
+//   marker pixels.
+// - Game.advectParticles: age, RK2 midpoint trace with SampleVelocity, survival test.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fl_internal.h"
+
+namespace fl {
+
+static constexpr int RB = 256;          // threads per reduction block
+static constexpr int RBLOCKS = 512;     // partials of the first pass
+
+struct MinMax {
+  float mn, mx;
+  long long imn, imx;  // index of the chosen element; -1 = the scan's initial value
+};
+
+__device__ __forceinline__ void mm_take_min(MinMax& a, float v, long long i) {
+  if (v < a.mn || (v == a.mn && i < a.imn)) {
+    a.mn = v;
+    a.imn = i;
+  }
+}
+__device__ __forceinline__ void mm_take_max(MinMax& a, float v, long long i) {
+  if (v > a.mx || (v == a.mx && i < a.imx)) {
+    a.mx = v;
+    a.imx = i;
+  }
+}
+__device__ __forceinline__ MinMax mm_init() { return MinMax{FLT_MAX, -FLT_MAX, -1, -1}; }
+
+// The sequential scan keeps the first element that compares below (above) the running
+// value, starting from +-MaxFloat32 (smoke.go:6-15): i.e. among the elements equal to the
+// extreme the lowest index, and the initial value when nothing beats it. `interior`: only
+// fluid interior cells take part (Vorticity, VelocityMagnitude, fluid.go:806-872).
+__device__ __forceinline__ void mm_block(MinMax m, MinMax* out) {
+  __shared__ MinMax red[RB];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      MinMax b = red[threadIdx.x + o];
+      MinMax& a = red[threadIdx.x];
+      if (b.imn >= 0) mm_take_min(a, b.mn, b.imn);
+      if (b.imx >= 0) mm_take_max(a, b.mx, b.imx);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+__global__ __launch_bounds__(RB) void k_minmax_partial(const float* __restrict__ f, const uint8_t* __restrict__ flags,
+                                                       int interior, size_t n, MinMax* __restrict__ part) {
+  MinMax m = mm_init();
+  for (size_t k = blockIdx.x * (size_t)RB + threadIdx.x; k < n; k += (size_t)gridDim.x * RB) {
+    if (interior) {
+      const uint32_t fl = flags[k];
+      if (!((fl & CF_INTERIOR) && (fl & CF_FLUID))) continue;
+    }
+    const float v = f[k];
+    mm_take_min(m, v, (long long)k);  // NaN and values tying the initial bound are never taken
+    mm_take_max(m, v, (long long)k);
+  }
+  mm_block(m, part + blockIdx.x);
+}
+__global__ __launch_bounds__(RB) void k_minmax_final(const MinMax* __restrict__ part, int nparts,
+                                                     float* __restrict__ out) {
+  MinMax m = mm_init();
+  for (int k = threadIdx.x; k < nparts; k += RB) {
+    const MinMax b = part[k];
+    if (b.imn >= 0) mm_take_min(m, b.mn, b.imn);
+    if (b.imx >= 0) mm_take_max(m, b.mx, b.imx);
+  }
+  __shared__ MinMax res;
+  mm_block(m, &res);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = res.imn >= 0 ? res.mn : FLT_MAX;
+    out[1] = res.imx >= 0 ? res.mx : -FLT_MAX;
+  }
+}
+
+// Go's uint8(float32): truncation toward zero, modulo 256 (values in range here).
+__device__ __forceinline__ uint32_t to_u8(float v) {
+  if (!(v > -2147483648.0f && v < 2147483648.0f)) return 0u;
+  return (uint32_t)(uint8_t)(long long)v;
+}
+__device__ __forceinline__ uint32_t rgba(float r, float g, float b) {
+  return to_u8(255.0f * r) | (to_u8(255.0f * g) << 8) | (to_u8(255.0f * b) << 16) | (0xffu << 24);
+}
+// getSciValue (colors.go:46-84)
+__device__ __forceinline__ uint32_t sci_color(float val, float mn, float mx) {
+  val = go_minf(go_maxf(val, mn), mx - 0.0001f);
+  const float d = mx - mn;
+  if (d <= 0.0f)
+    val = 0.5f;
+  else
+    val = (val - mn) / d;
+  const float m = 0.25f;
+  const float num = (float)floor((double)(val / m));
+  const float s = (val - num * m) / m;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  if (num == 0.0f) {
+    g = s;
+    b = 1.0f;
+  } else if (num == 1.0f) {
+    g = 1.0f;
+    b = 1.0f - s;
+  } else if (num == 2.0f) {
+    r = s;
+    g = 1.0f;
+  } else if (num == 3.0f) {
+    r = 1.0f;
+    g = 1.0f - s;
+  }
+  return rgba(r, g, b);
+}
+// getDivergingColor (colors.go:10-44)
+__device__ __forceinline__ uint32_t diverging_color(float val, float mn, float mx) {
+  float absMax;
+  if (mn != mn || mx != mx)
+    absMax = __int_as_float(0x7fc00000);
+  else
+    absMax = (float)fmax(fabs((double)mn), fabs((double)mx));
+  if (absMax < 1e-8f) return 0xffffffffu;
+  float t = val / absMax;
+  if (t > 1.0f) t = 1.0f;
+  if (t < -1.0f) t = -1.0f;
+  if (t >= 0.0f) return rgba(1.0f, 1.0f - t, 1.0f - t);
+  const float a = -t;
+  return rgba(1.0f - a, 1.0f - a, 1.0f);
+}
+
+// One pixel per thread: image (x = i, row j) shows fluid cell (i, NumY-1-j) (main.go:614-646);
+// solid cells are painted black afterwards in the reference, i.e. win (main.go:560-572).
+__global__ void k_render(int NX, int NY, const float* __restrict__ vals, const float* __restrict__ S,
+                         const float* __restrict__ range, int diverging, uint32_t* __restrict__ pix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= NX || j >= NY) return;
+  const int c = i * NY + (NY - 1 - j);
+  uint32_t px;
+  if (S[c] == 0.0f)
+    px = 0xff000000u;
+  else
+    px = diverging ? diverging_color(vals[c], range[0], range[1]) : sci_color(vals[c], range[0], range[1]);
+  pix[(size_t)j * NX + i] = px;
+}
+
+// Game.advectParticles (main.go:512-546) for one particle.
+__global__ void k_particles(Grid g, const float* __restrict__ U, const float* __restrict__ V,
+                            const float* __restrict__ S, float dt, int n, float* __restrict__ x,
+                            float* __restrict__ y, float* __restrict__ age, const float* __restrict__ maxAge,
+                            uint8_t* __restrict__ keep) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float a = age[k] + dt;
+  age[k] = a;
+  keep[k] = 0;
+  if (a > maxAge[k]) return;
+  int bad = 0;
+  float px = x[k], py = y[k];
+  const float u1 = sample_field(px, py, U, 0.0f, g.hs, g, &bad);
+  const float v1 = sample_field(px, py, V, g.hs, 0.0f, g, &bad);
+  const float mx = px + 0.5f * dt * u1;
+  const float my = py + 0.5f * dt * v1;
+  const float u2 = sample_field(mx, my, U, 0.0f, g.hs, g, &bad);
+  const float v2 = sample_field(mx, my, V, g.hs, 0.0f, g, &bad);
+  px = px + dt * u2;
+  py = py + dt * v2;
+  x[k] = px;
+  y[k] = py;
+  const float qx = px / g.h, qy = py / g.h;
+  if (!(qx > -2147483648.0f && qx < 2147483648.0f && qy > -2147483648.0f && qy < 2147483648.0f)) return;
+  const int fi = (int)qx, fj = (int)qy;  // truncation toward zero, as Go's int()
+  if (fi < 0 || fi >= g.NX || fj < 0 || fj >= g.NY) return;
+  keep[k] = S[fi * g.NY + fj] != 0.0f ? 1 : 0;
+}
+
+// ---- host side -------------------------------------------------------------------------
+static int rerr(int code, const std::string& m) {
+  set_error(m);
+  return code;
+}
+
+// Scratch of these calls: [NX*NY floats of view values][RBLOCKS MinMax][2 floats range]
+// [pixels or particle arrays], grown on demand.
+static int scratch(Ctx* c, size_t bytes, char** out) {
+  if (bytes > c->rs_cap) {
+    if (c->rs_mem) FL_HIP(hipFree(c->rs_mem));
+    c->rs_mem = nullptr;
+    FL_HIP(hipMalloc(&c->rs_mem, bytes));
+    c->rs_cap = bytes;
+  }
+  *out = c->rs_mem;
+  return FL_OK;
+}
+
+static int device_minmax(Ctx* c, const float* f, int interior, MinMax* part, float* range) {
+  hipLaunchKernelGGL(k_minmax_partial, dim3(RBLOCKS), dim3(RB), 0, c->stream, f, c->flags, interior, c->N, part);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(RB), 0, c->stream, part, RBLOCKS, range);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
+
+}  // namespace fl
+
+using namespace fl;
+
+extern "C" {
+
+int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
+  if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  float* f = nullptr;
+  switch (field) {
+    case FL_FIELD_U: f = c->U; break;
+    case FL_FIELD_V: f = c->V; break;
+    case FL_FIELD_NEWU: f = c->nU; break;
+    case FL_FIELD_NEWV: f = c->nV; break;
+    case FL_FIELD_P: f = c->p; break;
+    case FL_FIELD_S: f = c->S; break;
+    case FL_FIELD_M: f = c->M; break;
+    case FL_FIELD_NEWM: f = c->nM; break;
+    default: return rerr(FL_ERR_INVALID_ARG, "bad field");
+  }
+  int rc = ctx_prepare(c);
+  if (rc) return rc;
+  char* mem;
+  const size_t off = sizeof(MinMax) * RBLOCKS;
+  if ((rc = scratch(c, off + 2 * sizeof(float), &mem))) return rc;
+  if ((rc = device_minmax(c, f, 0, reinterpret_cast<MinMax*>(mem), reinterpret_cast<float*>(mem + off)))) return rc;
+  float r[2];
+  FL_HIP(hipMemcpyAsync(r, mem + off, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  if (mn) *mn = r[0];
+  if (mx) *mx = r[1];
+  return FL_OK;
+}
+
+int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) {
+  if (!ctx || !rgba) return rerr(FL_ERR_INVALID_ARG, "null argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (view < 0 || view > 3) return rerr(FL_ERR_INVALID_ARG, "view must be 0 smoke, 1 pressure, 2 speed, 3 vorticity");
+  if (nbytes != 4 * c->N) return rerr(FL_ERR_INVALID_ARG, "the image holds NumX*NumY RGBA pixels");
+  int rc = ctx_prepare(c);
+  if (rc) return rc;
+  char* mem;
+  const size_t o_part = 4 * c->N, o_range = o_part + sizeof(MinMax) * RBLOCKS, o_pix = o_range + 16;
+  if ((rc = scratch(c, o_pix + 4 * c->N, &mem))) return rc;
+  float* vals = reinterpret_cast<float*>(mem);
+  float* range = reinterpret_cast<float*>(mem + o_range);
+  uint32_t* pix = reinterpret_cast<uint32_t*>(mem + o_pix);
+  const float* f;
+  if (view == 0 || view == 1) {
+    f = view == 0 ? c->M : c->p;
+  } else {
+    if ((rc = launch_view(c, view == 2 ? 1 : 0, vals))) return rc;
+    f = vals;
+  }
+  if ((rc = device_minmax(c, f, view >= 2, reinterpret_cast<MinMax*>(mem + o_part), range))) return rc;
+  hipLaunchKernelGGL(k_render, dim3((c->NX + 255) / 256, c->NY), dim3(256), 0, c->stream, c->NX, c->NY, f, c->S,
+                     range, view == 3 ? 1 : 0, pix);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(rgba, pix, nbytes, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  if (overlay) {  // source and sink markers (main.go:574-595), in the reference's order
+    const long long stride = 4LL * c->NX, len = (long long)nbytes;
+    auto mark = [&](const Source& s, uint8_t r, uint8_t g) {
+      const long long idx = 4LL * s.I + (long long)(c->NY - s.J - 1) * stride;
+      if (idx >= 0 && idx + 3 < len) {
+        rgba[idx] = r;
+        rgba[idx + 1] = g;
+        rgba[idx + 2] = 0;
+        rgba[idx + 3] = 255;
+      }
+    };
+    for (const Source& s : c->sources) mark(s, 0, 255);
+    for (const Source& s : c->sinks) mark(s, 255, 0);
+  }
+  return FL_OK;
+}
+
+int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age, const float* max_age,
+                        uint8_t* keep) {
+  if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
+  if (n < 0 || (n > 0 && (!x || !y || !age || !max_age || !keep))) return rerr(FL_ERR_INVALID_ARG, "bad arguments");
+  if (n == 0) return FL_OK;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = ctx_prepare(c);
+  if (rc) return rc;
+  char* mem;
+  const size_t fb = 4 * (size_t)n;
+  if ((rc = scratch(c, 4 * fb + (size_t)n, &mem))) return rc;
+  float* dx = reinterpret_cast<float*>(mem);
+  float* dy = dx + n;
+  float* da = dy + n;
+  float* dm = da + n;
+  uint8_t* dk = reinterpret_cast<uint8_t*>(dm + n);
+  FL_HIP(hipMemcpyAsync(dx, x, fb, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipMemcpyAsync(dy, y, fb, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipMemcpyAsync(da, age, fb, hipMemcpyHostToDevice, c->stream));
+  FL_HIP(hipMemcpyAsync(dm, max_age, fb, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->grid, c->U, c->V, c->S, dt, n,
+                     dx, dy, da, dm, dk);
+  FL_HIP(hipGetLastError());
+  FL_HIP(hipMemcpyAsync(x, dx, fb, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipMemcpyAsync(y, dy, fb, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipMemcpyAsync(age, da, fb, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipMemcpyAsync(keep, dk, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  return FL_OK;
+}
+
+}  // extern "C"

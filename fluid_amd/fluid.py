@@ -250,6 +250,21 @@ class Fluid:
         """ApplyForce (fluid.go:761) weighted by a Gaussian of width sigma, truncated at 3 sigma."""
         check(self._L.fl_apply_force_gaussian(self._ctx, cx, cy, fx, fy, sigma))
 
+    def render(self, view, overlay=True):
+        """Game.Draw's RGBA image (NumY x NumX x 4): view 0 smoke, 1 pressure, 2 speed, 3 vorticity."""
+        pix = np.zeros((self.NumY, self.NumX, 4), dtype=np.uint8)
+        check(self._L.fl_render(self._ctx, int(view), int(bool(overlay)), pix.ctypes.data, pix.size))
+        return pix
+
+    def advect_particles(self, dt, x, y, age, max_age):
+        """Game.advectParticles (main.go:512-546) on float32 arrays, in place; returns keep."""
+        for a in (x, y, age, max_age):
+            assert a.dtype == np.float32 and a.flags.c_contiguous and a.size == x.size
+        keep = np.zeros(x.size, dtype=np.uint8)
+        check(self._L.fl_advect_particles(self._ctx, dt, x.size, x.ctypes.data, y.ctypes.data, age.ctypes.data,
+                                          max_age.ctypes.data, keep.ctypes.data))
+        return keep
+
     def SetCircularObstacle(self, cx, cy, radius):
         check(self._L.fl_set_circular_obstacle(self._ctx, cx, cy, radius))
 

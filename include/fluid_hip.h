@@ -143,6 +143,19 @@ int fl_velocity_magnitude(fl_ctx* ctx, float* out, size_t n, float* min_value, f
 /* Pressure() (pressure.go:5) / Smoke() (smoke.go:5): MinValue/MaxValue over all cells. */
 int fl_field_minmax(fl_ctx* ctx, int field, float* min_value, float* max_value);
 
+/* Game.Draw's image (main.go:549-652, colors.go:10-84): NumY rows x NumX RGBA8 pixels, row j
+ * showing fluid row NumY-1-j; view 0 Smoke(), 1 Pressure(), 2 VelocityMagnitude() with the
+ * scientific colormap (getSciValue), 3 Vorticity() with the diverging one (getDivergingColor);
+ * solid cells black; with `overlay` source pixels green and sink pixels red. Computed on the
+ * device, field ranges included: nbytes = 4*NumX*NumY. */
+int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes);
+/* Game.advectParticles (main.go:512-546) for n particles in one call: age += dt; a particle
+ * older than max_age, or whose cell int(x/h), int(y/h) after the RK2 midpoint step (with
+ * SampleVelocity, fluid.go:799) is outside the grid or solid, gets keep = 0. x, y, age are
+ * updated in place; the caller drops the particles with keep = 0 (keeping the order). */
+int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age, const float* max_age,
+                        uint8_t* keep);
+
 /* Unexported passes, for white-box tests in the manner of fluid_test.go. */
 typedef enum fl_pass_id {
   FL_PASS_COPY_BORDER_UV = 0,   /* copyBorder(newU,U); copyBorder(newV,V)  fluid.go:436 */

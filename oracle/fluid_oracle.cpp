@@ -1014,6 +1014,136 @@ template <class T> struct Scene {
       }
     }
   }
+  // ---- particles and rendering (main.go:512-546, 549-652; colors.go:10-84) -------------
+  // Game.advectParticles: age, drop if too old, RK2 midpoint with SampleVelocity, drop if the
+  // cell int(x/h), int(y/h) is outside the grid or solid. keep[k] = 1 for survivors.
+  void advectParticles(T dt, int n, T* x, T* y, T* age, const T* maxAge, uint8_t* keep) {
+    const T h = f->h;
+    for (int k = 0; k < n; k++) {
+      age[k] += dt;
+      keep[k] = 0;
+      if (age[k] > maxAge[k]) continue;
+      T u1, v1, u2, v2;
+      f->SampleVelocity(x[k], y[k], &u1, &v1);
+      T midX = x[k] + (T)0.5 * dt * u1;
+      T midY = y[k] + (T)0.5 * dt * v1;
+      f->SampleVelocity(midX, midY, &u2, &v2);
+      x[k] += dt * u2;
+      y[k] += dt * v2;
+      T qx = x[k] / h, qy = y[k] / h;
+      if (!(qx > (T)-2147483648.0 && qx < (T)2147483648.0 && qy > (T)-2147483648.0 && qy < (T)2147483648.0))
+        continue;
+      int fi = (int)qx, fj = (int)qy;  // Go int() truncates toward zero
+      if (fi < 0 || fi >= f->NX || fj < 0 || fj >= f->NY) continue;
+      if (f->S[fi * f->NY + fj] == 0) continue;
+      keep[k] = 1;
+    }
+  }
+  static uint8_t u8(float v) {  // Go uint8(float32): truncation toward zero, modulo 256
+    if (!(v > -2147483648.0f && v < 2147483648.0f)) return 0;
+    return (uint8_t)(int64_t)v;
+  }
+  static void sciColor(float val, float mn, float mx, uint8_t* px) {
+    val = go_min(go_max(val, mn), mx - 0.0001f);
+    float d = mx - mn;
+    if (d <= 0)
+      val = 0.5f;
+    else
+      val = (val - mn) / d;
+    const float m = 0.25f;
+    float num = (float)std::floor((double)(val / m));
+    float s = (val - num * m) / m;
+    float r = 0, g = 0, b = 0;
+    if (num == 0) {
+      g = s;
+      b = 1;
+    } else if (num == 1) {
+      g = 1;
+      b = 1 - s;
+    } else if (num == 2) {
+      r = s;
+      g = 1;
+    } else if (num == 3) {
+      r = 1;
+      g = 1 - s;
+    }
+    px[0] = u8(255 * r);
+    px[1] = u8(255 * g);
+    px[2] = u8(255 * b);
+    px[3] = 0xff;
+  }
+  static void divergingColor(float val, float mn, float mx, uint8_t* px) {
+    float absMax = (float)std::fmax(std::fabs((double)mn), std::fabs((double)mx));
+    if (std::isnan((double)mn) || std::isnan((double)mx)) absMax = std::numeric_limits<float>::quiet_NaN();
+    if (absMax < 1e-8f) {
+      px[0] = px[1] = px[2] = px[3] = 255;
+      return;
+    }
+    float t = val / absMax;
+    if (t > 1) t = 1;
+    if (t < -1) t = -1;
+    float r, g, b;
+    if (t >= 0) {
+      r = 1.0f;
+      g = 1.0f - t;
+      b = 1.0f - t;
+    } else {
+      float a = -t;
+      r = 1.0f - a;
+      g = 1.0f - a;
+      b = 1.0f;
+    }
+    px[0] = u8(255 * r);
+    px[1] = u8(255 * g);
+    px[2] = u8(255 * b);
+    px[3] = 0xff;
+  }
+  // Game.Draw's image (main.go:549-612): view 0 smoke, 1 pressure, 2 velocity magnitude (sci
+  // colormap), 3 vorticity (diverging); image row j shows fluid row NumY-1-j; then solid cells
+  // black and, with `overlay`, source pixels green and sink pixels red.
+  void render(int view, bool overlay, uint8_t* pix) {
+    const int NX = f->NX, NY = f->NY, stride = 4 * NX, len = 4 * NX * NY;
+    std::vector<T> vals(f->numCells);
+    T mn, mx;
+    if (view == 0 || view == 1) {
+      const std::vector<T>& src = view == 0 ? f->M : f->p;
+      vals = src;
+      f->FieldMinMax(view == 0 ? F_M : F_P, &mn, &mx);
+    } else if (view == 2) {
+      f->VelocityMagnitude(vals.data(), &mn, &mx);
+    } else {
+      f->Vorticity(vals.data(), &mn, &mx);
+    }
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++) {
+        float v = (float)vals[i * NY + (NY - j - 1)];
+        uint8_t* px = pix + 4 * i + j * stride;
+        if (view == 3)
+          divergingColor(v, (float)mn, (float)mx, px);
+        else
+          sciColor(v, (float)mn, (float)mx, px);
+      }
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        if (f->S[i * NY + (NY - j - 1)] == 0) {
+          uint8_t* px = pix + 4 * i + j * stride;
+          px[0] = px[1] = px[2] = 0;
+          px[3] = 0xff;
+        }
+    if (!overlay) return;
+    auto mark = [&](const Source& s, uint8_t r, uint8_t g) {
+      int idx = 4 * s.I + (NY - s.J - 1) * stride;
+      if (idx >= 0 && idx + 3 < len) {
+        pix[idx] = r;
+        pix[idx + 1] = g;
+        pix[idx + 2] = 0;
+        pix[idx + 3] = 255;
+      }
+    };
+    for (auto& s : sources) mark(s, 0, 255);
+    for (auto& s : sinks) mark(s, 255, 0);
+  }
+
   // main.go:453-467 (smoke brush: AddSmoke(strength*0.3) over a disc)
   void smokeBrush(int ci, int cj, int r, T strength) {
     for (int i = ci - r; i <= ci + r; i++)
@@ -1292,6 +1422,18 @@ double orc_rollout(void* h, int steps, float dt, const void* Fu, const void* Fv,
     return loss;
   });
   return 0.0;
+}
+// Particles in float32 (the reference's Particle fields are float32).
+void orc_advect_particles(void* h, float dt, int n, float* x, float* y, float* age, const float* maxAge,
+                          uint8_t* keep) {
+  OrcHandle* H = (OrcHandle*)h;
+  if (H->prec != 32) return;
+  H->s32.advectParticles(dt, n, x, y, age, maxAge, keep);
+}
+void orc_render(void* h, int view, int overlay, uint8_t* pix) {
+  OrcHandle* H = (OrcHandle*)h;
+  if (H->prec != 32) return;
+  H->s32.render(view, overlay != 0, pix);
 }
 void orc_set_circular_obstacle(void* h, int cx, int cy, int r) {
   OrcHandle* H = (OrcHandle*)h;

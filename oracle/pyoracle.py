@@ -95,6 +95,8 @@ def lib():
             "orc_scene_draw_line": (None, [vp, i, i, i, i, i]),
             "orc_scene_smoke_brush": (None, [vp, i, i, i, f]),
             "orc_rollout": (d, [vp, i, f, vp, vp, vp, vp, vp]),
+            "orc_advect_particles": (None, [vp, f, i, vp, vp, vp, vp, vp]),
+            "orc_render": (None, [vp, i, i, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -190,6 +192,24 @@ class OracleFluid:
         ip = (ctypes.c_int * 5)(int(self.UseMultigrid), int(self.MultigridLevels), int(self.UseBFECC),
                                 int(self.PressureIters), int(self.EarlyExit))
         self._L.orc_set_params(self._h, fp, ip)
+
+    # ---- particles and rendering (float32 handles) ----
+    def advect_particles(self, dt, x, y, age, max_age):
+        """Game.advectParticles (main.go:512-546) on float32 arrays, in place; returns keep."""
+        assert self.prec == 32
+        for a in (x, y, age, max_age):
+            assert a.dtype == np.float32 and a.flags.c_contiguous
+        keep = np.zeros(x.size, dtype=np.uint8)
+        self._L.orc_advect_particles(self._h, dt, x.size, x.ctypes.data, y.ctypes.data, age.ctypes.data,
+                                     max_age.ctypes.data, keep.ctypes.data)
+        return keep
+
+    def render(self, view, overlay=True):
+        """Game.Draw's RGBA image (NumY rows x NumX columns x 4), main.go:549-652."""
+        assert self.prec == 32
+        pix = np.zeros((self.NumY, self.NumX, 4), dtype=np.uint8)
+        self._L.orc_render(self._h, int(view), int(bool(overlay)), pix.ctypes.data)
+        return pix
 
     # ---- differentiable rollout (oracle/fluid_adjoint.inc) ----
     def rollout(self, steps, dt, Fu, Fv, target, grad=True):

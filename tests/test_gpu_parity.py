@@ -283,3 +283,80 @@ def test_multigrid_pass_and_converged_break():
         f.Simulate(0.1)
     assert_same(g, o, "quiescent")
     assert g.stats()["sweeps_run"] == o.last_sweeps() == 3
+
+
+def test_field_ranges_follow_the_scan():
+    """Pressure()/Smoke() ranges (smoke.go:6-15) on the device: the first element in index
+    order wins among equal values (+0 vs -0), NaN is never selected, +-MaxFloat32 when
+    nothing beats the initial bounds."""
+    g, o = pair(1000.0, 40, 30, 0.01)
+    rng = np.random.default_rng(3)
+    M = rng.standard_normal((42, 32)).astype(np.float32)
+    p = np.zeros((42, 32), np.float32)
+    p[5, 5], p[7, 3], p[9, 1] = -0.0, 0.0, np.nan   # all-zero field: the first zero is -0
+    M[3, 4] = np.nan
+    for f in (g, o):
+        f.set("M", M)
+        f.set("p", p)
+    for fn in ("Smoke", "Pressure"):
+        a, b = getattr(g, fn)(), getattr(o, fn)()
+        gm = np.array([a.MinValue, a.MaxValue], dtype=np.float32)
+        om = np.array([b.MinValue, b.MaxValue], dtype=np.float32)
+        assert gm.view(np.uint32).tolist() == om.view(np.uint32).tolist(), (fn, gm, om)
+    # nothing beats the initial bounds: an all-NaN field
+    for f in (g, o):
+        f.set("p", np.full((42, 32), np.nan, np.float32))
+    a, b = g.Pressure(), o.Pressure()
+    assert (a.MinValue, a.MaxValue) == (b.MinValue, b.MaxValue) == (float(np.finfo(np.float32).max),
+                                                                     -float(np.finfo(np.float32).max))
+
+
+@pytest.mark.parametrize("view", [0, 1, 2, 3], ids=["smoke", "pressure", "speed", "vorticity"])
+def test_render_matches_oracle(view):
+    g = C.jet_scene(lambda *a: Fluid(*a), 90, 70, 20, jet_rows=10)
+    o = C.jet_scene(lambda *a: OracleFluid(*a), 90, 70, 20, jet_rows=10)
+    for f in (g, o):
+        f.SetCircularObstacle(40, 35, 6)
+        f.add_source(10, 60, 0.5, 0.0)
+        f.add_sink(80, 10, 0.2, 0.1)
+        f.add_source(300, 5, 0.0, 0.0)   # outside the grid: its marker index wraps as in the reference
+        f.UseBFECC, f.Confinement = True, 0.2
+    for _ in range(6):
+        g.update(C.DT)
+        o.update(C.DT)
+    assert_same(g, o, "state")
+    a, b = g.render(view), o.render(view)
+    assert a.shape == b.shape == (g.NumY, g.NumX, 4)
+    bad = np.argwhere((a != b).any(axis=2))
+    assert len(bad) == 0, (len(bad), bad[:3], a[tuple(bad[0])], b[tuple(bad[0])])
+
+
+def test_particles_match_oracle():
+    g = C.jet_scene(lambda *a: Fluid(*a), 80, 64, 20, jet_rows=12)
+    o = C.jet_scene(lambda *a: OracleFluid(*a), 80, 64, 20, jet_rows=12)
+    for f in (g, o):
+        f.SetCircularObstacle(30, 32, 8)
+    for _ in range(5):
+        g.update(C.DT)
+        o.update(C.DT)
+    rng = np.random.default_rng(9)
+    n = 5000
+    h = np.float32(C.H)
+    x = (rng.random(n) * 84 - 1).astype(np.float32) * h   # some start outside the grid
+    y = (rng.random(n) * 68 - 1).astype(np.float32) * h
+    age = (rng.random(n) * 1.4).astype(np.float32)
+    mx = np.full(n, 1.5, np.float32)
+    gx, gy, ga, gm = x.copy(), y.copy(), age.copy(), mx.copy()
+    ox, oy, oa, om = x.copy(), y.copy(), age.copy(), mx.copy()
+    for frame in range(4):
+        kg = g.advect_particles(np.float32(C.DT), gx, gy, ga, gm)
+        ko = o.advect_particles(np.float32(C.DT), ox, oy, oa, om)
+        assert np.array_equal(kg, ko), frame
+        for a, b in ((gx, ox), (gy, oy), (ga, oa)):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), frame
+        keep = kg.astype(bool)
+        assert 0 < keep.sum() < len(keep)
+        gx, gy, ga, gm = [np.ascontiguousarray(a[keep]) for a in (gx, gy, ga, gm)]
+        ox, oy, oa, om = [np.ascontiguousarray(a[keep]) for a in (ox, oy, oa, om)]
+        g.update(C.DT)
+        o.update(C.DT)

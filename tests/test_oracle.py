@@ -414,3 +414,69 @@ def test_fp32_tracks_fp64(bfecc, conf, tol):
         a, b = outs[0][k].astype(np.float64), outs[1][k]
         scale = max(np.abs(b).max(), 1e-30)
         assert np.abs(a - b).max() / scale < tol, k
+
+
+# ---- application layer: colormaps and particles (main/colors.go, main.go:512-652) --------
+
+def test_sci_colormap_closed_forms():
+    # getSciValue (colors.go:46-84): min -> blue, the four quarter points, max clamped below red
+    f = fluid(4, 1)  # 6 x 3 cells
+    M = np.zeros((6, 3), F32)
+    M[0, :] = [0.0, 0.25, 0.5]
+    M[1, :] = [0.75, 1.0, 0.125]
+    f.set("M", M)
+    pix = f.render(0, overlay=False)        # image row j shows fluid row NumY-1-j
+    px = lambda i, j: tuple(pix[f.NumY - 1 - j, i])  # noqa: E731
+    assert px(0, 0) == (0, 0, 255, 255)          # val 0: num 0, s 0
+    assert px(0, 1) == (0, 255, 255, 255)        # val 0.25: num 1, s 0 -> cyan
+    assert px(0, 2) == (0, 255, 0, 255)          # val 0.5: num 2, s 0 -> green
+    assert px(1, 0) == (255, 255, 0, 255)        # val 0.75: num 3, s 0 -> yellow
+    # val 1 = max: clamped to 1 - 0.0001, num 3, s ~ 0.9996 -> (255, 0, 0) after truncation
+    assert px(1, 1) == (255, int(F32(255) * (F32(1) - F32((F32(1) - F32(1e-4) - F32(0.75)) / F32(0.25)))), 0, 255)
+    assert px(1, 2) == (0, 127, 255, 255)        # val 0.125: num 0, s 0.5 -> g = uint8(127.5)
+
+
+def test_diverging_colormap_and_solid_overlay():
+    f = fluid(6, 6, hh=1.0)
+    U = np.zeros((8, 8), F32)
+    U[3, 4] = 1.0   # curl at (3,3) and (3,5) has opposite signs
+    f.set("U", U)
+    S = f.get("S")
+    S[6, 6] = 0
+    f.set("S", S)
+    vals = f.Vorticity()
+    vmax = max(abs(vals.MinValue), abs(vals.MaxValue))
+    pix = f.render(3, overlay=False)
+    px = lambda i, j: tuple(pix[f.NumY - 1 - j, i])  # noqa: E731
+    v = vals.values.reshape(8, 8)
+    for (i, j) in [(3, 3), (3, 5), (2, 2)]:
+        t = np.clip(F32(v[i, j]) / F32(vmax), -1, 1)
+        if t >= 0:
+            exp = (255, int(F32(255) * (F32(1) - t)), int(F32(255) * (F32(1) - t)), 255)
+        else:
+            a = -t
+            exp = (int(F32(255) * (F32(1) - a)), int(F32(255) * (F32(1) - a)), 255, 255)
+        assert px(i, j) == exp, (i, j, px(i, j), exp)
+    assert px(2, 2) == (255, 255, 255, 255)      # zero curl -> white
+    assert px(6, 6) == (0, 0, 0, 255)            # solid cell painted black
+
+
+def test_particles_in_uniform_flow():
+    # RK2 midpoint in a uniform field moves by exactly dt*u; age and survival rules
+    f = fluid(10, 10, hh=1.0)
+    f.set("U", np.full((12, 12), 0.5, F32))
+    f.set("V", np.full((12, 12), -0.25, F32))
+    x = np.array([3.5, 3.5, 10.9, 5.0], F32)
+    y = np.array([4.5, 4.5, 5.0, 0.2], F32)
+    age = np.array([0.0, 14.95, 0.0, 0.0], F32)
+    mx = np.array([15.0, 15.0, 15.0, 15.0], F32)
+    keep = f.advect_particles(F32(0.1), x, y, age, mx)
+    assert x[0] == F32(3.5) + F32(0.1) * F32(0.5) and y[0] == F32(4.5) + F32(0.1) * F32(-0.25)
+    assert keep.tolist() == [1, 0, 1, 1]            # particle 1 exceeded MaxAge
+    assert age[1] == F32(14.95) + F32(0.1)
+    S = f.get("S")
+    nx, ny = x[2] + F32(0.1) * F32(0.5), y[2] + F32(0.1) * F32(-0.25)
+    S[int(nx), int(ny)] = 0                        # the cell it reaches next is solid: dropped
+    f.set("S", S)
+    keep = f.advect_particles(F32(0.1), x, y, age, mx)
+    assert keep[2] == 0 and x[2] == nx

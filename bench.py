@@ -205,6 +205,7 @@ def bench_diff(args):
     iters = args.iters if args.iters != 200 else 80
     f = C.diff_scene(lambda *a: Fluid(*a), size, size, iters)
     f.Confinement = args.confinement
+    f.BlockSweeps = args.block_sweeps
     shape = (f.NumX, f.NumY)
     T = C.gaussian_blobs(f.NumX, f.NumY)
     z = np.zeros(shape, dtype=np.float32)

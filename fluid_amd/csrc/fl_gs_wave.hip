@@ -487,10 +487,14 @@ static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, 
       }
 }
 
-int pick_tk(const fl_params& P, int N) {
+// Sweeps per tile: the knob if set; otherwise 16 when the 16-deep tiling yields enough tiles
+// to keep every CU busy (wider blocks of sweeps halve the handoff traffic), else 8 (twice the
+// tiles, shorter pipeline fill). `tiles16` counts the tiles at depth 16.
+int pick_tk(const fl_params& P, int N, long long tiles16) {
   if (P.block_sweeps == 8 || P.block_sweeps == 16) return P.block_sweeps;
-  return N >= 64 ? 16 : 8;
+  return (N >= 64 && tiles16 >= 512) ? 16 : 8;
 }
+static long long tiles_at16(int NX, int N) { return (long long)((NX + N - 1 + 63) / 64) * ((N + 15) / 16); }
 
 template <int TK> static int occupancy_grid(int dev, int ntiles, bool many) {
   int per_cu = 0;
@@ -505,7 +509,7 @@ template <int TK> static int occupancy_grid(int dev, int ntiles, bool many) {
 
 static int layout(Ctx* c, int N) {
   GsWork& W = c->gs;
-  const int tk = pick_tk(c->prm, N);
+  const int tk = pick_tk(c->prm, N, tiles_at16(c->NX, N));
   if (W.n_alloc == N && W.tk == tk) return FL_OK;
   FL_HIP(hipStreamSynchronize(c->stream));  // earlier solves may still use the old layout
   pressure_free(c);
@@ -591,7 +595,9 @@ int pressure_free(Ctx* c) {
 // All members' tiles in one ticket list, merged by hyperplane start; one persistent launch
 // drains it (and one more for the members that early-exit). Results land in U2, V2, T3.
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
-  const int tk = pick_tk(B->prm, N);
+  long long t16 = 0;
+  for (const SimDesc& d : descs) t16 += tiles_at16(d.g.NX, N);
+  const int tk = pick_tk(B->prm, N, t16);
   auto fill = [&]() {
     for (size_t e = 0; e < descs.size(); e++) {
       const SimDesc& f = B->gs_fields[e];

@@ -35,9 +35,8 @@ def workloads():
     return [
         ("jet 256x256, SL, 40 sweeps", lambda mk: C.jet_scene(mk, 256, 256, 40), 200,
          (1, 2, 5, 10, 20, 50, 100, 150, 200)),
-        ("karman 512x256, BFECC, confinement 0.3, 100 sweeps",
-         lambda mk: C.karman_scene(mk, 512, 256, 100, radius=16, centre=(64, 128), band=8), 30,
-         (1, 2, 5, 10, 20, 30)),
+        ("karman 2048x1024, BFECC, confinement 0.3, 100 sweeps", lambda mk: C.karman_scene(mk), 10,
+         (1, 2, 5, 10)),
         ("cavity 512x512, BFECC, 200 sweeps", lambda mk: C.cavity_scene(mk, 512, 512, 200), 16, (1, 2, 4, 8, 16)),
     ]
 

@@ -93,7 +93,8 @@ __device__ __forceinline__ const SimDesc& sim_desc(const ManySims& m, int e) { r
 template <int TK, class M>
 __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
   const int lane = threadIdx.x & (LANES - 1);
-  const int wave = threadIdx.x / LANES;
+  // wave-uniform, and held in a scalar register so every test on it is a scalar branch
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES);
   const int tid = threadIdx.x;
   constexpr int NT = TK * LANES;
 
@@ -366,14 +367,16 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
           st_dev(kr + (LANES + 1) + lane + 1, Vb);
           st_dev(kr + 2 * (LANES + 1) + lane + 1, p);
         }
-        if (last && real_row && j >= 0 && j <= NY - 1) {
-          const size_t c = (size_t)row * NY + j;
-          const bool tu = row >= 1 && j >= 1 && j <= NY - 2;
-          const bool tv = row >= 1 && row <= NX - 2 && j >= 1;
-          const bool tp = row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2;
-          if (tu) Uout[c] = Ul;
-          if (tv) Vout[c] = Vb;
-          if (tp) Pout[c] = p;
+        if (last) {  // only the wave of the final sweep writes the results
+          if (real_row && j >= 0 && j <= NY - 1) {
+            const size_t c = (size_t)row * NY + j;
+            const bool tu = row >= 1 && j >= 1 && j <= NY - 2;
+            const bool tv = row >= 1 && row <= NX - 2 && j >= 1;
+            const bool tp = row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2;
+            if (tu) Uout[c] = Ul;
+            if (tv) Vout[c] = Vb;
+            if (tp) Pout[c] = p;
+          }
         }
         __syncthreads();
       }

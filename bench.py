@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--solver", type=int, default=0)
     ap.add_argument("--block-sweeps", type=int, default=0)
+    ap.add_argument("--precise", action="store_true",
+                    help="PreciseAdvection: back-trace in cell units (an addition; default is the reference's arithmetic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--calibrate", action="store_true",
@@ -303,6 +305,7 @@ def main():
     f = build(args.config, lambda *a: Fluid(*a), args.size, args.iters)
     f.Solver = args.solver
     f.BlockSweeps = args.block_sweeps
+    f.PreciseAdvection = args.precise
     W, H = f.NumX - 2, f.NumY - 2
     cells = W * H
     for _ in range(args.warmup):
@@ -348,7 +351,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "%s %dx%d FP32, %d GS sweeps/step, BFECC %s, dt 0.1" % (
             args.config, W, H, sweeps, "on" if f.UseBFECC else "off"), "grid": [W, H], "pressure_sweeps": sweeps,
-            "solver": ["auto", "hyperplane", "temporal-blocked"][args.solver]},
+            "solver": ["auto", "hyperplane", "temporal-blocked"][args.solver],
+            "advection": "precise (cell-unit back-trace)" if args.precise else "reference arithmetic"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, args.size, args.iters),
                      "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},

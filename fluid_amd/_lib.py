@@ -29,6 +29,7 @@ class FlParams(ctypes.Structure):
         ("early_exit", ctypes.c_int32),
         ("solver", ctypes.c_int32),
         ("block_sweeps", ctypes.c_int32),
+        ("precise_advection", ctypes.c_int32),
     ]
 
 

@@ -163,6 +163,39 @@ __device__ __forceinline__ SGeo sample_geo(float x, float y, const float* __rest
   s.ddy = (sx * (d - a) + tx * (c - b)) * g.h1 * my;
   return s;
 }
+// The same at (i + dX, j + dY) in cell units (precise_advection): the derivative in cells
+// times h1, 0 on an axis whose position was clamped.
+__device__ __forceinline__ SGeo sample_geo_cells(int i, int j, float dX, float dY, const float* __restrict__ f,
+                                                 bool hx, bool hy, const Grid& g) {
+  int bad = 0, x0, y0;
+  float tx, ty;
+  const float mx = axis_cells(i, dX, g.NX, hx, x0, tx, &bad) ? 1.0f : 0.0f;
+  const float my = axis_cells(j, dY, g.NY, hy, y0, ty, &bad) ? 1.0f : 0.0f;
+  const int x1 = min(x0 + 1, g.NX - 1), y1 = min(y0 + 1, g.NY - 1);
+  const float sx = 1.0f - tx, sy = 1.0f - ty;
+  const int n = g.NY;
+  SGeo s;
+  s.k[0] = x0 * n + y0;
+  s.k[1] = x1 * n + y0;
+  s.k[2] = x1 * n + y1;
+  s.k[3] = x0 * n + y1;
+  s.w[0] = sx * sy;
+  s.w[1] = tx * sy;
+  s.w[2] = tx * ty;
+  s.w[3] = sx * ty;
+  const float a = f[s.k[0]], b = f[s.k[1]], c = f[s.k[2]], d = f[s.k[3]];
+  s.val = sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
+  s.ddx = (sy * (b - a) + ty * (c - d)) * g.h1 * mx;
+  s.ddy = (sx * (d - a) + tx * (c - b)) * g.h1 * my;
+  return s;
+}
+// A back-traced sample from (i,j): (x,y) is the reference's absolute position, offset plus
+// (du,dv) = the trace step times the velocity; precise_advection samples at (du,dv)*h1 cells.
+__device__ __forceinline__ SGeo trace_geo(float x, float y, int i, int j, float du, float dv,
+                                          const float* __restrict__ f, float dx, float dy, const Grid& g) {
+  if (g.precise) return sample_geo_cells(i, j, du * g.h1, dv * g.h1, f, dx != 0.0f, dy != 0.0f, g);
+  return sample_geo(x, y, f, dx, dy, g);
+}
 __device__ __forceinline__ void scatter(const SGeo& s, float lam, float* __restrict__ dst) {
   for (int q = 0; q < 4; q++) aadd(dst + s.k[q], lam * s.w[q]);
 }
@@ -199,9 +232,10 @@ __global__ void k_adj_adv_vel(Grid g, const uint8_t* __restrict__ flags, const f
     if (lam != 0.0f) {
       float x = (float)i * g.h;
       float y = (float)j * g.h + g.ha;
-      x = x + s * Ut[c];
-      y = y + s * avg_v4(Vt, c, n);
-      const SGeo q = sample_geo(x, y, Su, 0.0f, g.hs, g);
+      const float du = s * Ut[c], dv = s * avg_v4(Vt, c, n);
+      x = x + du;
+      y = y + dv;
+      const SGeo q = trace_geo(x, y, i, j, du, dv, Su, 0.0f, g.hs, g);
       scatter(q, lam, lSU);
       aadd(lTU + c, lam * q.ddx * s);
       const float gy = lam * q.ddy * s * 0.25f;
@@ -218,9 +252,10 @@ __global__ void k_adj_adv_vel(Grid g, const uint8_t* __restrict__ flags, const f
     if (lam != 0.0f) {
       float x = (float)i * g.h + g.ha;
       float y = (float)j * g.h;
-      x = x + s * avg_u4(Ut, c, n);
-      y = y + s * Vt[c];
-      const SGeo q = sample_geo(x, y, Sv, g.hs, 0.0f, g);
+      const float du = s * avg_u4(Ut, c, n), dv = s * Vt[c];
+      x = x + du;
+      y = y + dv;
+      const SGeo q = trace_geo(x, y, i, j, du, dv, Sv, g.hs, 0.0f, g);
       scatter(q, lam, lSV);
       const float gx = lam * q.ddx * s * 0.25f;
       aadd(lTU + c - 1, gx);
@@ -252,7 +287,7 @@ __global__ void k_adj_adv_smoke(Grid g, const uint8_t* __restrict__ flags, const
     const float v = (V[c] + V[c + 1]) * 0.5f * SA;
     const float x = (float)i * g.h + g.hs + s * u;
     const float y = (float)j * g.h + g.hs + s * v;
-    const SGeo q = sample_geo(x, y, Sm, g.hs, g.hs, g);
+    const SGeo q = trace_geo(x, y, i, j, s * u, s * v, Sm, g.hs, g.hs, g);
     if (relu && !(q.val > 0.0f)) return;
     scatter(q, lam, lS);
     const float gu = lam * q.ddx * s * 0.5f * SA, gv = lam * q.ddy * s * 0.5f * SA;
@@ -322,9 +357,10 @@ __global__ void k_adj_corr_vel(Grid g, const uint8_t* __restrict__ flags, const 
       if (a) {
         float x = (float)i * g.h;
         float y = (float)j * g.h + g.ha;
-        x = x + dt * oU[c];
-        y = y + dt * avg_v4(oV, c, n);
-        q = sample_geo(x, y, fU, 0.0f, g.hs, g);
+        const float du = dt * oU[c], dv = dt * avg_v4(oV, c, n);
+        x = x + du;
+        y = y + dv;
+        q = trace_geo(x, y, i, j, du, dv, fU, 0.0f, g.hs, g);
         bwd = q.val;
       }
       const float o = oU[c];
@@ -356,9 +392,10 @@ __global__ void k_adj_corr_vel(Grid g, const uint8_t* __restrict__ flags, const 
       if (a) {
         float x = (float)i * g.h + g.ha;
         float y = (float)j * g.h;
-        x = x + dt * avg_u4(oU, c, n);
-        y = y + dt * oV[c];
-        q = sample_geo(x, y, fV, g.hs, 0.0f, g);
+        const float du = dt * avg_u4(oU, c, n), dv = dt * oV[c];
+        x = x + du;
+        y = y + dv;
+        q = trace_geo(x, y, i, j, du, dv, fV, g.hs, 0.0f, g);
         bwd = q.val;
       }
       const float o = oV[c];
@@ -407,7 +444,7 @@ __global__ void k_adj_corr_smoke(Grid g, const uint8_t* __restrict__ flags, cons
     const float v = (V[c] + V[c + 1]) * 0.5f * SA;
     const float x = (float)i * g.h + g.hs + dt * u;
     const float y = (float)j * g.h + g.hs + dt * v;
-    q = sample_geo(x, y, fM, g.hs, g.hs, g);
+    q = trace_geo(x, y, i, j, dt * u, dt * v, fM, g.hs, g.hs, g);
     bwd = q.val;
   }
   const float o = M1[c];
@@ -763,7 +800,7 @@ static int adj_step(Vjp* J, const float* tp, int K, float dt) {
   Ctx* c = J->c;
   const fl_params& P = c->prm;
   const size_t N = J->N;
-  const Grid g = c->grid;
+  const Grid g = c->desc().g;
   const dim3 gr = grid_of(c), bl(BX, BY);
   hipStream_t st = c->stream;
   float** L = J->lam;

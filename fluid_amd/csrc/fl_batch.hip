@@ -166,6 +166,7 @@ static int prepare_batch(Batch* B, float dt) {
     d[e] = c->desc();
     d[e].cp = c->density * c->h / dt;
     d[e].early_exit = B->prm.early_exit;
+    d[e].g.precise = B->prm.precise_advection != 0;
     d[e].nmax = MEMBER_NMAX;
   }
   // the batch's own tile layout replaces whatever GS workspace a member holds from

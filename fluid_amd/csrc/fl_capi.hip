@@ -296,6 +296,7 @@ fl_params default_params() {
   p.early_exit = 1;
   p.solver = 0;
   p.block_sweeps = 0;
+  p.precise_advection = 0;
   return p;
 }
 

@@ -17,6 +17,7 @@ struct Grid {
   float ha;       // h/2                     fluid.go:298 (advect face offset)
   float xmax;     // float32(NumX)*h         fluid.go:363
   float ymax;     // float32(NumY)*h         fluid.go:364
+  int precise;    // back-trace in cell units (sample_cells) instead of absolute positions
 };
 
 // Cell flag bits (uint8 per cell), derived from S (S must be exactly 0.0 or 1.0).
@@ -127,6 +128,69 @@ __device__ __forceinline__ float sample_field(float x, float y, const float* __r
     x1 = max(x1, 0);
     y1 = max(y1, 0);
   }
+  float sx = 1.0f - tx;
+  float sy = 1.0f - ty;
+  const int n = g.NY;
+  float a = f[x0 * n + y0], b = f[x1 * n + y0], c = f[x1 * n + y1], d = f[x0 * n + y1];
+  return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
+}
+
+// Back-trace in cell units (the precise_advection parameter; the reference always samples
+// at absolute positions). Every back-trace of the step lands at X = i + dX, Y = j + dY in
+// cell units (the face and sample offsets cancel), dX = (s*u)*h1 for the trace step
+// s = -dt or +dt. The reference forms x = i*h + s*u and rounds it at ulp(N*h), ~N*6e-8
+// cells; here floor(d) is split off and the fraction d - floor(d) is exact, so the
+// position carries only dX's own rounding. The clamp x in [h, N*h] becomes X in
+// [1 - o, N - o], o = 1/2 on an axis with a half-cell sample offset (y of U, x of V, both
+// of M). X = N exactly maps to sampleField's x0 = N-1, tx = 1. Returns whether X was
+// inside the clamp range.
+__device__ __forceinline__ bool axis_cells(int base, float d, int N, bool half, int& i0, float& t, int* bad) {
+  if (d != d) {  // a NaN velocity: the reference flags the index and samples NaN
+    *bad = 1;
+    i0 = 0;
+    t = d;
+    return false;
+  }
+  bool inside = true;
+  const int loI = half ? 0 : 1, hiI = half ? N - 1 : N;
+  const float loF = half ? 0.5f : 0.0f, hiF = loF;
+  const float fl = floorf(d);
+  if (fl < -16777216.0f) {
+    i0 = loI;
+    t = loF;
+    inside = false;
+  } else if (fl > 16777216.0f) {
+    i0 = hiI;
+    t = hiF;
+    inside = false;
+  } else {
+    i0 = base + (int)fl;
+    t = d - fl;
+    if (i0 < loI || (i0 == loI && t < loF)) {
+      i0 = loI;
+      t = loF;
+      inside = false;
+    } else if (i0 > hiI || (i0 == hiI && t > hiF)) {
+      i0 = hiI;
+      t = hiF;
+      inside = false;
+    }
+  }
+  if (i0 > N - 1) {
+    i0 = N - 1;
+    t = 1.0f;
+  }
+  return inside;
+}
+// The bilinear sample of sampleField (fluid.go:389-397) at (i + dX, j + dY); hx/hy: the
+// axis carries a half-cell sample offset.
+__device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, const float* __restrict__ f, bool hx,
+                                              bool hy, const Grid& g, int* bad) {
+  int x0, y0;
+  float tx, ty;
+  axis_cells(i, dX, g.NX, hx, x0, tx, bad);
+  axis_cells(j, dY, g.NY, hy, y0, ty, bad);
+  const int x1 = min(x0 + 1, g.NX - 1), y1 = min(y0 + 1, g.NY - 1);
   float sx = 1.0f - tx;
   float sy = 1.0f - ty;
   const int n = g.NY;

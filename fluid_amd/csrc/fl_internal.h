@@ -182,6 +182,7 @@ struct Ctx {
   SimDesc desc() const {
     SimDesc d{};
     d.g = grid;
+    d.g.precise = prm.precise_advection != 0;
     float* b[NBUF] = {U, V, nU, nV, p, S, M, nM, U2, V2, M2, T0, T1, T2, T3};
     for (int k = 0; k < NBUF; k++) d.buf[k] = b[k];
     d.flags = flags;

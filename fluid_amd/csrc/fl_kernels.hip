@@ -359,7 +359,8 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
     float v = avg_v(Vt, c, n);
     x = x - dt * u;
     y = y - dt * v;
-    ou = sample_field(x, y, Ut, 0.0f, g.hs, g, &badl);
+    ou = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ut, false, true, g, &badl)
+                   : sample_field(x, y, Ut, 0.0f, g.hs, g, &badl);
   } else {
     ou = ring ? Ut[c] : nU[c];
   }
@@ -370,7 +371,8 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
     float v = Vt[c];
     x = x - dt * u;
     y = y - dt * v;
-    ov = sample_field(x, y, Vt, g.hs, 0.0f, g, &badl);
+    ov = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Vt, true, false, g, &badl)
+                   : sample_field(x, y, Vt, g.hs, 0.0f, g, &badl);
   } else {
     ov = ring ? Vt[c] : nV[c];
   }
@@ -429,7 +431,8 @@ __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, i
     float v = avg_v(Vo, c, n);
     x = x + dt * u;
     y = y + dt * v;
-    bu = sample_field(x, y, Uf, 0.0f, g.hs, g, &badl);
+    bu = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Uf, false, true, g, &badl)
+                   : sample_field(x, y, Uf, 0.0f, g.hs, g, &badl);
   } else {
     bu = ring ? Uf[c] : 0.0f;
   }
@@ -440,7 +443,8 @@ __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, i
     float v = Vo[c];
     x = x + dt * u;
     y = y + dt * v;
-    bv = sample_field(x, y, Vf, g.hs, 0.0f, g, &badl);
+    bv = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Vf, true, false, g, &badl)
+                   : sample_field(x, y, Vf, g.hs, 0.0f, g, &badl);
   } else {
     bv = ring ? Vf[c] : 0.0f;
   }
@@ -487,7 +491,8 @@ template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float d
     float v = (V[c] + V[c + 1]) * 0.5f * smokeAdv;
     float x = (float)i * g.h + g.hs - dt * u;
     float y = (float)j * g.h + g.hs - dt * v;
-    float val = sample_field(x, y, Ms, g.hs, g.hs, g, &badl);
+    float val = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ms, true, true, g, &badl)
+                          : sample_field(x, y, Ms, g.hs, g.hs, g, &badl);
     if (visc > 0.0f) {
       float sd = visc * 0.3f * dt;
       float nb = Ms[c - n] + Ms[c + n] + Ms[c - 1] + Ms[c + 1] - 4.0f * Ms[c];
@@ -534,7 +539,8 @@ template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int
       float v = (V[c] + V[c + 1]) * 0.5f * smokeAdv;
       float x = (float)i * g.h + g.hs + dt * u;
       float y = (float)j * g.h + g.hs + dt * v;
-      bm = sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
+      bm = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Mf, true, true, g, &badl)
+                     : sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
       if (badl) atomicOr(d->ctrl + 2, 1);
     }
     cm = om - (bm - om) * 0.5f;

@@ -83,6 +83,9 @@ def _init_knobs(o):
     o.Relaxation = None  # None: follow the module-level Relaxation like the Go package var
     o.Solver = 0         # 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel
     o.BlockSweeps = 0
+    # Addition: back-trace in cell units (no reference counterpart; off = the reference's
+    # absolute-position arithmetic, bit for bit).
+    o.PreciseAdvection = False
 
 
 def _params_of(o):
@@ -100,6 +103,7 @@ def _params_of(o):
     p.early_exit = int(bool(o.EarlyExit))
     p.solver = int(o.Solver)
     p.block_sweeps = int(o.BlockSweeps)
+    p.precise_advection = int(bool(o.PreciseAdvection))
     return p
 
 

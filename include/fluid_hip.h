@@ -69,6 +69,7 @@ typedef struct fl_params {
   int32_t early_exit;        /* 1: stop after the first sweep with maxDiv < 1e-5 */
   int32_t solver;            /* 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel */
   int32_t block_sweeps;      /* sweeps per temporal block (0 = tuned default) */
+  int32_t precise_advection; /* 1: back-trace in cell units (no reference counterpart), default 0 */
 } fl_params;
 
 /* Statistics of the last Simulate (no reference counterpart). */

@@ -17,6 +17,10 @@
 //   pkg/fluid/pressure.go, smoke.go   ScalarField min/max conventions
 //   main/main.go:200-249,474-486,747-858   presets, sources, jet, walls, strokes
 //
+// Additions with no reference counterpart, pinned by tests/test_precise_oracle.py against the
+// reference sampling above (bit-equal where positions are exact in float32, equal in FP64 to
+// rounding): PreciseAdvection (axisCells, sampleCells, disp), the back-trace in cell units.
+//
 // Third-party arithmetic: Go 1.24 `math.Sin` (pure Go, Cephes sin.c algorithm) is
 // restated in go_sin() from the published Cephes method; it is pinned by the tests
 // against the C library's sin over the whole argument domain the step uses.
@@ -157,6 +161,8 @@ template <class T> struct Fluid {
   // Additions: Simulate hard-codes 8 sweeps (fluid.go:81) and always early-exits.
   int PressureIters = 8;
   bool EarlyExit = true;
+  // Addition (off in the reference): back-trace in cell units, see sampleCells.
+  bool PreciseAdvection = false;
 
   int threads = 1;
 
@@ -385,6 +391,69 @@ template <class T> struct Fluid {
            sx * ty * data[x0 * n + y1];
   }
 
+  // PreciseAdvection (an addition; the reference always samples at absolute positions).
+  // Every back-trace of the step lands at X = i + dX, Y = j + dY in cell units (the face
+  // and sample offsets cancel), with dX = (s*u)*h1 for the trace step s = -dt or +dt. The
+  // reference forms x = i*h + s*u and rounds it at ulp(N*h), i.e. to ~N*6e-8 cells; here
+  // the integer part is split off exactly and only the fraction d - floor(d) is kept, so
+  // the position error is that of dX alone. The clamp x in [h, N*h] becomes X in
+  // [1 - o, N - o], o = 1/2 on an axis with a half-cell sample offset (y of U, x of V,
+  // both of M); corners, weights and their evaluation order are sampleField's.
+  // Returns whether X was inside [1 - o, N - o] (not clamped).
+  bool axisCells(int base, T d, int N, bool half, int& i0, T& t) {
+    if (d != d) {  // a NaN velocity: the reference flags the index and samples NaN
+      invalidIndex = true;
+      i0 = 0;
+      t = d;
+      return false;
+    }
+    bool inside = true;
+    const int loI = half ? 0 : 1, hiI = half ? N - 1 : N;
+    const T loF = half ? (T)0.5 : (T)0, hiF = half ? (T)0.5 : (T)0;
+    const T fl = std::floor(d);
+    if (fl < (T)-16777216) {
+      i0 = loI;
+      t = loF;
+      inside = false;
+    } else if (fl > (T)16777216) {
+      i0 = hiI;
+      t = hiF;
+      inside = false;
+    } else {
+      i0 = base + (int)fl;
+      t = d - fl;
+      if (i0 < loI || (i0 == loI && t < loF)) {
+        i0 = loI;
+        t = loF;
+        inside = false;
+      } else if (i0 > hiI || (i0 == hiI && t > hiF)) {
+        i0 = hiI;
+        t = hiF;
+        inside = false;
+      }
+    }
+    if (i0 > N - 1) {  // X = N exactly: sampleField's min(x0, N-1) with tx = 1
+      i0 = N - 1;
+      t = (T)1;
+    }
+    return inside;
+  }
+  T sampleCells(int i, int j, T dX, T dY, const std::vector<T>& data, int kind) {
+    const int n = NY;
+    int x0, y0;
+    T tx, ty;
+    axisCells(i, dX, NX, kind != SK_U, x0, tx);
+    axisCells(j, dY, NY, kind != SK_V, y0, ty);
+    const int x1 = std::min(x0 + 1, NX - 1), y1 = std::min(y0 + 1, NY - 1);
+    T sx = (T)1 - tx;
+    T sy = (T)1 - ty;
+    return sx * sy * data[x0 * n + y0] + tx * sy * data[x1 * n + y0] + tx * ty * data[x1 * n + y1] +
+           sx * ty * data[x0 * n + y1];
+  }
+  // Displacement in cells of a trace step s (-dt forward, +dt for BFECC's backward pass):
+  // s*u is the reference's dt*u with the trace's sign, exactly.
+  T disp(T s, T u) const { return (s * u) * ((T)1 / h); }
+
   // fluid.go:291-333
   void advectVelocity(T dt) {
     copyBorder(newU, U);
@@ -401,7 +470,7 @@ template <class T> struct Fluid {
           T v = avgV(V, i, j);
           x = x - dt * u;
           y = y - dt * v;
-          newU[c] = sample(x, y, U, SK_U);
+          newU[c] = PreciseAdvection ? sampleCells(i, j, disp(-dt, u), disp(-dt, v), U, SK_U) : sample(x, y, U, SK_U);
         }
         if (S[c] != 0 && S[c - 1] != 0 && i < NX - 1) {
           T x = (T)i * h + h2;
@@ -410,7 +479,7 @@ template <class T> struct Fluid {
           T v = V[c];
           x = x - dt * u;
           y = y - dt * v;
-          newV[c] = sample(x, y, V, SK_V);
+          newV[c] = PreciseAdvection ? sampleCells(i, j, disp(-dt, u), disp(-dt, v), V, SK_V) : sample(x, y, V, SK_V);
         }
       }
     });
@@ -431,7 +500,7 @@ template <class T> struct Fluid {
           T v = (V[c] + V[c + 1]) * (T)0.5 * SmokeAdvection;
           T x = (T)i * h + h2 - dt * u;
           T y = (T)j * h + h2 - dt * v;
-          T val = sample(x, y, M, SK_M);
+          T val = PreciseAdvection ? sampleCells(i, j, disp(-dt, u), disp(-dt, v), M, SK_M) : sample(x, y, M, SK_M);
           if (ViscosityDiffusion > 0) {
             T smokeDiffusion = ViscosityDiffusion * (T)0.3f * dt;
             T nb = M[c - n] + M[c + n] + M[c - 1] + M[c + 1] - (T)4 * M[c];
@@ -689,7 +758,7 @@ template <class T> struct Fluid {
           T v = avgV(V, i, j);
           x = x + dt * u;
           y = y + dt * v;
-          bwdU[c] = sample(x, y, fwdU, SK_U);
+          bwdU[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdU, SK_U) : sample(x, y, fwdU, SK_U);
         }
         if (S[c] != 0 && S[c - 1] != 0 && i < NX - 1) {
           T x = (T)i * h + h2;
@@ -698,7 +767,7 @@ template <class T> struct Fluid {
           T v = V[c];
           x = x + dt * u;
           y = y + dt * v;
-          bwdV[c] = sample(x, y, fwdV, SK_V);
+          bwdV[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdV, SK_V) : sample(x, y, fwdV, SK_V);
         }
       }
     });
@@ -739,7 +808,7 @@ template <class T> struct Fluid {
           T v = (V[c] + V[c + 1]) * (T)0.5 * SmokeAdvection;
           T x = (T)i * h + h2 + dt * u;
           T y = (T)j * h + h2 + dt * v;
-          bwdM[c] = sample(x, y, fwdM, SK_M);
+          bwdM[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdM, SK_M) : sample(x, y, fwdM, SK_M);
         }
       }
     });
@@ -1266,6 +1335,7 @@ void orc_set_params(void* h, const double* fp, const int* ip) {
     F->UseBFECC = ip[2] != 0;
     F->PressureIters = ip[3];
     F->EarlyExit = ip[4] != 0;
+    F->PreciseAdvection = ip[5] != 0;
   });
 }
 void orc_set_threads(void* h, int n) {

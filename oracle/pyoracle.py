@@ -147,7 +147,8 @@ class OracleFluid:
     """CPU restatement of pkg/fluid's Fluid; prec=32 is float32-faithful, 64 is the FP64 model."""
 
     KNOBS = ("Confinement", "ViscosityDiffusion", "PressureDamping", "TurbulenceStrength", "SmokeAdvection",
-             "Relaxation", "UseMultigrid", "MultigridLevels", "UseBFECC", "PressureIters", "EarlyExit")
+             "Relaxation", "UseMultigrid", "MultigridLevels", "UseBFECC", "PressureIters", "EarlyExit",
+             "PreciseAdvection")
 
     def __init__(self, density, width, height, h, prec=32, threads=1):
         self._L = lib()
@@ -170,6 +171,7 @@ class OracleFluid:
         self.UseBFECC = False
         self.PressureIters = 8
         self.EarlyExit = True
+        self.PreciseAdvection = False
         self._L.orc_set_threads(self._h, threads)
 
     New = classmethod(lambda cls, density, width, height, h, **kw: cls(density, width, height, h, **kw))
@@ -189,8 +191,8 @@ class OracleFluid:
     def _push(self):
         fp = (ctypes.c_double * 6)(self.Confinement, self.ViscosityDiffusion, self.PressureDamping,
                                    self.TurbulenceStrength, self.SmokeAdvection, self.Relaxation)
-        ip = (ctypes.c_int * 5)(int(self.UseMultigrid), int(self.MultigridLevels), int(self.UseBFECC),
-                                int(self.PressureIters), int(self.EarlyExit))
+        ip = (ctypes.c_int * 6)(int(self.UseMultigrid), int(self.MultigridLevels), int(self.UseBFECC),
+                                int(self.PressureIters), int(self.EarlyExit), int(self.PreciseAdvection))
         self._L.orc_set_params(self._h, fp, ip)
 
     # ---- particles and rendering (float32 handles) ----

@@ -200,24 +200,30 @@ static int enqueue_step(Batch* B, float dt) {
     if ((rc = batch_viscosity(B, dt))) return rc;
     if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
   }
+  // The velocity after the solve stays in (U2, V2): confinement, turbulence and borders work
+  // there in place and the (first) velocity advection writes the step's result into (U, V),
+  // so only p and, after semi-Lagrangian smoke, M are copied back.
+  int cu = B_U, cv = B_V;
   if (P.pressure_iters > 0) {
     if ((rc = batch_gs_solve(B, P.pressure_iters))) return rc;
-    if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2)) ||
-        (rc = batch_copy_role(B, B_P, B_T3)))
-      return rc;
+    if ((rc = batch_copy_role(B, B_P, B_T3))) return rc;
+    cu = B_U2;
+    cv = B_V2;
   }
-  if ((rc = batch_conf_turb(B, dt))) return rc;
-  if ((rc = batch_borders(B))) return rc;
+  if ((rc = batch_conf_turb(B, dt, cu, cv))) return rc;
+  if ((rc = batch_borders(B, cu, cv))) return rc;
+  // forward (or only) pass: from the current velocity into the other pair of roles
+  const int fu = cu == B_U ? B_U2 : B_U, fv = cv == B_V ? B_V2 : B_V;
   if (P.use_bfecc) {
-    if ((rc = batch_advect_velocity(B, B_U, B_V, B_U2, B_V2, dt))) return rc;
-    if ((rc = batch_bfecc_velocity_corr(B, dt))) return rc;
+    if ((rc = batch_advect_velocity(B, cu, cv, fu, fv, dt))) return rc;
+    if ((rc = batch_bfecc_velocity_corr(B, cu, cv, fu, fv, dt))) return rc;
     if ((rc = batch_advect_velocity(B, B_T0, B_T1, B_U, B_V, dt))) return rc;
     if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
     if ((rc = batch_bfecc_smoke_corr(B, dt))) return rc;
     return batch_advect_smoke(B, B_T2, B_M, dt);
   }
-  if ((rc = batch_advect_velocity(B, B_U, B_V, B_U2, B_V2, dt))) return rc;
-  if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
+  if ((rc = batch_advect_velocity(B, cu, cv, fu, fv, dt))) return rc;
+  if (fu != B_U && ((rc = batch_copy_role(B, B_U, fu)) || (rc = batch_copy_role(B, B_V, fv)))) return rc;
   if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
   return batch_copy_role(B, B_M, B_M2);
 }

@@ -291,10 +291,10 @@ int batch_fill_role(Batch* B, int r, float v);
 int batch_copy_role(Batch* B, int rdst, int rsrc);
 int batch_flags(Batch* B);
 int batch_viscosity(Batch* B, float dt);
-int batch_conf_turb(Batch* B, float dt);
-int batch_borders(Batch* B);
+int batch_conf_turb(Batch* B, float dt, int rU, int rV);
+int batch_borders(Batch* B, int rU, int rV);
 int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt);
-int batch_bfecc_velocity_corr(Batch* B, float dt);
+int batch_bfecc_velocity_corr(Batch* B, int rUo, int rVo, int rUf, int rVf, float dt);
 int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt);
 int batch_bfecc_smoke_corr(Batch* B, float dt);
 int batch_apply_sources(Batch* B);

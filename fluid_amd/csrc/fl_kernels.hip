@@ -210,32 +210,37 @@ __device__ __forceinline__ float curl_at(const float* __restrict__ U, const floa
   return dvdx - dudy;
 }
 
-template <class M> __global__ void k_curl(M m, int rcurl) {
+// (rU, rV): buffer roles holding the current velocity (U, V for a context; the solve's outputs
+// in a batched step).
+template <class M> __global__ void k_curl(M m, int rcurl, int rU, int rV) {
   const SimDesc* d;
   int i, j;
   if (!cell_of(m, d, i, j)) return;
   const int n = d->g.NY, c = i * n + j;
   const uint32_t f = d->flags[c];
   d->buf[rcurl][c] =
-      ((f & CF_INTERIOR) && (f & CF_FLUID)) ? curl_at(d->buf[B_U], d->buf[B_V], c, n, d->g.h) : 0.0f;
+      ((f & CF_INTERIOR) && (f & CF_FLUID)) ? curl_at(d->buf[rU], d->buf[rV], c, n, d->g.h) : 0.0f;
 }
 int launch_curl(Ctx* c, float* curl) {
   const int r = role_of(c, curl);
-  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) { hipLaunchKernelGGL(k_curl<OneSim>, g, b, 0, s, m, r); });
+  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) {
+    hipLaunchKernelGGL(k_curl<OneSim>, g, b, 0, s, m, r, (int)B_U, (int)B_V);
+  });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 template <class M>
-__global__ void k_conf_turb(M m, int rcurl, float conf, float dt, float turb, int do_conf, int do_turb) {
+__global__ void k_conf_turb(M m, int rU, int rV, int rcurl, float conf, float dt, float turb, int do_conf,
+                            int do_turb) {
   const SimDesc* d;
   int i, j;
   if (!cell_of(m, d, i, j)) return;
   const int n = d->g.NY, c = i * n + j;
   const uint32_t f = d->flags[c];
   if (!((f & CF_INTERIOR) && (f & CF_FLUID))) return;
-  float* U = d->buf[B_U];
-  float* V = d->buf[B_V];
+  float* U = d->buf[rU];
+  float* V = d->buf[rV];
   const float* curl = d->buf[rcurl];
   const float h = d->g.h;
   float u = U[c], v = V[c];
@@ -268,21 +273,21 @@ int launch_conf_turb(Ctx* c, const float* curl, float dt, int do_conf, int do_tu
   const float turb = c->prm.turbulence_strength * dt, conf = c->prm.confinement;
   const int r = role_of(c, curl);
   launch_cells(c, [&](const OneSim& m, dim3 g, dim3 b, hipStream_t s) {
-    hipLaunchKernelGGL(k_conf_turb<OneSim>, g, b, 0, s, m, r, conf, dt, turb, do_conf, do_turb);
+    hipLaunchKernelGGL(k_conf_turb<OneSim>, g, b, 0, s, m, (int)B_U, (int)B_V, r, conf, dt, turb, do_conf, do_turb);
   });
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
 // ---- handleBorders (fluid.go:236-289) -----------------------------------------------------
-template <class M> __global__ void k_borders(M m) {
+template <class M> __global__ void k_borders(M m, int rU, int rV) {
   const SimDesc* d;
   int t;
   ring_of(m, d, t);
   const int n = d->g.NY, NX = d->g.NX, NY = d->g.NY;
   const float* S = d->buf[B_S];
-  float* U = d->buf[B_U];
-  float* V = d->buf[B_V];
+  float* U = d->buf[rU];
+  float* V = d->buf[rV];
   if (t < NX) {
     int i = t;
     if (S[i * n + 0] == 0.0f || S[i * n + 1] == 0.0f) {
@@ -322,7 +327,7 @@ int launch_borders(Ctx* c) {
   if (c->NX < 3 || c->NY < 3) return FL_ERR_UNSUPPORTED;
   const int m = std::max(c->NX, c->NY);
   OneSim map{c->desc()};
-  hipLaunchKernelGGL(k_borders<OneSim>, dim3((m + 255) / 256), dim3(256), 0, c->stream, map);
+  hipLaunchKernelGGL(k_borders<OneSim>, dim3((m + 255) / 256), dim3(256), 0, c->stream, map, (int)B_U, (int)B_V);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -584,18 +589,18 @@ int batch_viscosity(Batch* B, float dt) {
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
-int batch_conf_turb(Batch* B, float dt) {
+int batch_conf_turb(Batch* B, float dt, int rU, int rV) {
   const fl_params& P = B->prm;
   const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
-  if (conf) hipLaunchKernelGGL(k_curl<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_T0);
+  if (conf) hipLaunchKernelGGL(k_curl<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_T0, rU, rV);
   if (conf || turb)
-    hipLaunchKernelGGL(k_conf_turb<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_T0, P.confinement, dt,
+    hipLaunchKernelGGL(k_conf_turb<ManySims>, FL_BATCH_CELLS(B), many(B), rU, rV, (int)B_T0, P.confinement, dt,
                        P.turbulence_strength * dt, (int)conf, (int)turb);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
-int batch_borders(Batch* B) {
-  hipLaunchKernelGGL(k_borders<ManySims>, FL_BATCH_RING(B), many(B));
+int batch_borders(Batch* B, int rU, int rV) {
+  hipLaunchKernelGGL(k_borders<ManySims>, FL_BATCH_RING(B), many(B), rU, rV);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -604,9 +609,9 @@ int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
-int batch_bfecc_velocity_corr(Batch* B, float dt) {
-  hipLaunchKernelGGL(k_bfecc_velocity_corr<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_U, (int)B_V, (int)B_U2,
-                     (int)B_V2, (int)B_T0, (int)B_T1, dt);
+int batch_bfecc_velocity_corr(Batch* B, int rUo, int rVo, int rUf, int rVf, float dt) {
+  hipLaunchKernelGGL(k_bfecc_velocity_corr<ManySims>, FL_BATCH_CELLS(B), many(B), rUo, rVo, rUf, rVf, (int)B_T0,
+                     (int)B_T1, dt);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

@@ -10,7 +10,8 @@
 // Backward: per segment, the checkpoint is restored and the segment recomputed with a
 // tape of the step's intermediate fields, then each step is reversed:
 //   smoke BFECC / SL       scatter adjoint of the bilinear samples (atomics), the trace's
-//                          velocity dependence, the 3x3 clamp and max(.,0) branches
+//                          velocity dependence, the diffusion term, the 3x3 clamp and
+//                          max(.,0) branches
 //   velocity BFECC / SL    same, with the back-trace and final pass tracing the field they
 //                          sample
 //   borders, turbulence, confinement (curl, normalised grad |curl|, speed factor)
@@ -21,9 +22,12 @@
 //                          the temporal-blocked solver itself runs the adjoint; the p output
 //                          of that run (cp = 1) is each cell's sum of updates, which gives
 //                          the adjoint of the faces next to solid cells.
+//   viscosity              the Laplacian step transposed (a gather); faces the advection
+//                          does not advect keep the viscosity's newU/newV, so their adjoint
+//                          joins it there.
 //   sources and force      the force's gradient is the adjoint entering Simulate; cells set
 //                          by SetVelocity lose theirs.
-// Viscosity and multigrid are not differentiated (FL_ERR_UNSUPPORTED).
+// The multigrid solver is not differentiated (FL_ERR_UNSUPPORTED).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -45,7 +49,7 @@ enum TapeField { T_U2 = 0, T_V2, T_U3, T_V3, T_U5, T_V5, T_FU, T_FV, T_CU, T_CV,
 // adjoint work fields
 enum LamField {
   L_U = 0, L_V, L_M, L_CU, L_CV, L_FU, L_FV, L_OU, L_OV, L_CM, L_FM, L_M1, L_CURL, L_LCD, L_GX, L_GY, L_LCURL,
-  L_GFU, L_GFV, NLAM
+  L_GFU, L_GFV, L_SU, L_SV, NLAM
 };
 // saved state: U, V, newU, newV, p, M, newM
 static constexpr int NSTATE = 7;
@@ -215,12 +219,14 @@ __device__ __forceinline__ bool adv_v(uint32_t f, int i, int j, int NX) {
 // Adjoint of a semi-Lagrangian velocity pass (advectVelocity, fluid.go:291-333, or BFECC's
 // backward trace): trace with (Ut,Vt) and step s, sample (Su,Sv). lO: adjoint of the output;
 // the sampled field's adjoint is added to lS, the tracing field's to lT; `ring`: non-advected
-// ring faces are copies of the input (copyBorder).
+// ring faces are copies of the input (copyBorder). stU/stV (null without viscosity): the
+// other non-advected faces keep newU/newV, which applyViscosity set this step; their adjoint
+// is added there (each thread writes only its own face).
 __global__ void k_adj_adv_vel(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ Ut,
                               const float* __restrict__ Vt, const float* __restrict__ Su,
                               const float* __restrict__ Sv, float s, const float* __restrict__ lOU,
                               const float* __restrict__ lOV, float* lSU, float* lSV, float* lTU, float* lTV,
-                              int ring) {
+                              int ring, float* __restrict__ stU, float* __restrict__ stV) {
   const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
   const int NX = g.NX, NY = g.NY, n = NY;
   if (i >= NX || j >= NY) return;
@@ -246,6 +252,8 @@ __global__ void k_adj_adv_vel(Grid g, const uint8_t* __restrict__ flags, const f
     }
   } else if (ring && onring) {
     aadd(lSU + c, lOU[c]);
+  } else if (ring && stU) {
+    stU[c] += lOU[c];
   }
   if (adv_v(f, i, j, NX)) {
     const float lam = lOV[c];
@@ -266,15 +274,20 @@ __global__ void k_adj_adv_vel(Grid g, const uint8_t* __restrict__ flags, const f
     }
   } else if (ring && onring) {
     aadd(lSV + c, lOV[c]);
+  } else if (ring && stV) {
+    stV[c] += lOV[c];
   }
 }
 
-// Adjoint of a smoke pass (advectSmoke, fluid.go:400-434, no diffusion; or BFECC's
-// backward trace): trace with the face-averaged (U,V) * SA and step s, sample Sm,
-// max(.,0) when `relu`. `ring`: the ring is a copy of the input.
+// Adjoint of a smoke pass (advectSmoke, fluid.go:400-434; or BFECC's backward trace):
+// trace with the face-averaged (U,V) * SA and step s, sample Sm, max(.,0) when `relu`.
+// `ring`: the ring is a copy of the input. `diffuse`: advectSmoke's diffusion
+// sd * (4-neighbour Laplacian of Sm), sd = ViscosityDiffusion * 0.3 * dt, is added before the
+// max, in the forward's order so the clamp takes the same branch.
 __global__ void k_adj_adv_smoke(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ U,
                                 const float* __restrict__ V, const float* __restrict__ Sm, float s, float SA,
-                                int relu, const float* __restrict__ lO, float* lS, float* lU, float* lV, int ring) {
+                                int relu, const float* __restrict__ lO, float* lS, float* lU, float* lV, int ring,
+                                int diffuse, float sd) {
   const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
   const int NX = g.NX, NY = g.NY, n = NY;
   if (i >= NX || j >= NY) return;
@@ -288,8 +301,21 @@ __global__ void k_adj_adv_smoke(Grid g, const uint8_t* __restrict__ flags, const
     const float x = (float)i * g.h + g.hs + s * u;
     const float y = (float)j * g.h + g.hs + s * v;
     const SGeo q = trace_geo(x, y, i, j, s * u, s * v, Sm, g.hs, g.hs, g);
-    if (relu && !(q.val > 0.0f)) return;
+    float val = q.val;
+    if (diffuse) {
+      const float nb = Sm[c - n] + Sm[c + n] + Sm[c - 1] + Sm[c + 1] - 4.0f * Sm[c];
+      val += sd * nb;
+    }
+    if (relu && !(val > 0.0f)) return;
     scatter(q, lam, lS);
+    if (diffuse) {
+      const float ls = lam * sd;
+      aadd(lS + c - n, ls);
+      aadd(lS + c + n, ls);
+      aadd(lS + c - 1, ls);
+      aadd(lS + c + 1, ls);
+      aadd(lS + c, ls * -4.0f);
+    }
     const float gu = lam * q.ddx * s * 0.5f * SA, gv = lam * q.ddy * s * 0.5f * SA;
     aadd(lU + c, gu);
     aadd(lU + c + n, gu);
@@ -663,6 +689,36 @@ __global__ void k_mirror_out(Grid g, const float* __restrict__ S, const float* _
 
 // Force and sources transposed: dL/dF += lambda on fluid interior cells, then the cells
 // written by SetVelocity lose their adjoint.
+// applyViscosity (fluid.go:112-142) transposed, as a gather: on fluid interior cells the
+// forward sets new = u + visc * (sum of the 4 neighbours - 4u), elsewhere new = u. The
+// adjoint of new is lO + st (st: faces that kept newU/newV through the advection).
+__global__ void k_adj_visc(Grid g, const uint8_t* __restrict__ flags, float visc, const float* __restrict__ lOU,
+                           const float* __restrict__ lOV, const float* __restrict__ stU,
+                           const float* __restrict__ stV, float* __restrict__ oU, float* __restrict__ oV) {
+  const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+  const int NX = g.NX, NY = g.NY, n = NY;
+  if (i >= NX || j >= NY) return;
+  const int c = i * n + j;
+  auto upd = [&](int k) { return (flags[k] & CF_INTERIOR) && (flags[k] & CF_FLUID); };
+  const float lu = lOU[c] + stU[c], lv = lOV[c] + stV[c];
+  float ou = lu, ov = lv;
+  if (upd(c)) {
+    ou += lu * -4.0f * visc;
+    ov += lv * -4.0f * visc;
+  }
+  const int nb[4] = {i > 0 ? c - n : -1, i < NX - 1 ? c + n : -1, j > 0 ? c - 1 : -1, j < NY - 1 ? c + 1 : -1};
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int k = nb[q];
+    if (k >= 0 && upd(k)) {
+      ou += (lOU[k] + stU[k]) * visc;
+      ov += (lOV[k] + stV[k]) * visc;
+    }
+  }
+  oU[c] = ou;
+  oV[c] = ov;
+}
+
 __global__ void k_adj_force(Grid g, const uint8_t* __restrict__ flags, const float* __restrict__ lU,
                             const float* __restrict__ lV, float* __restrict__ gU, float* __restrict__ gV) {
   const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
@@ -731,6 +787,7 @@ static int vjp_step(Vjp* J, float dt, float* tp, int* kept) {
                      J->Fv);
   FL_HIP(hipGetLastError());
   if ((rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;
+  if (P.viscosity_diffusion > 0.0f && (rc = launch_viscosity(c, dt))) return rc;
   if ((rc = pressure_solve(c, P.pressure_iters, dt))) return rc;
   if (tp && ((rc = dcopy(J, tp + T_U2 * N, c->U)) || (rc = dcopy(J, tp + T_V2 * N, c->V)))) return rc;
   const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
@@ -807,36 +864,43 @@ static int adj_step(Vjp* J, const float* tp, int K, float dt) {
   auto T = [&](int k) { return tp + (size_t)k * N; };
   auto zero = [&](int k) { return launch_fill(L[k], 0.0f, N, st); };
   const float SA = P.smoke_advection;
+  const bool visc = P.viscosity_diffusion > 0.0f;
+  const int dif = visc ? 1 : 0;
+  const float sd = P.viscosity_diffusion * 0.3f * dt;  // advectSmoke's diffusion weight
   int rc;
   // smoke: adds the final velocity's share to L_U, L_V
   if ((rc = zero(L_M1))) return rc;
   if (P.use_bfecc) {
     if ((rc = zero(L_CM)) || (rc = zero(L_FM))) return rc;
     hipLaunchKernelGGL(k_adj_adv_smoke, gr, bl, 0, st, g, c->flags, T(T_U6), T(T_V6), T(T_CM), -dt, SA, 1, L[L_M],
-                       L[L_CM], L[L_U], L[L_V], 1);
+                       L[L_CM], L[L_U], L[L_V], 1, dif, sd);
     hipLaunchKernelGGL(k_adj_corr_smoke, gr, bl, 0, st, g, c->flags, T(T_M1), T(T_FM), T(T_U6), T(T_V6), dt, SA,
                        L[L_CM], L[L_M1], L[L_FM], L[L_U], L[L_V]);
     hipLaunchKernelGGL(k_adj_adv_smoke, gr, bl, 0, st, g, c->flags, T(T_U6), T(T_V6), T(T_M1), -dt, SA, 1, L[L_FM],
-                       L[L_M1], L[L_U], L[L_V], 1);
+                       L[L_M1], L[L_U], L[L_V], 1, dif, sd);
   } else {
     hipLaunchKernelGGL(k_adj_adv_smoke, gr, bl, 0, st, g, c->flags, T(T_U6), T(T_V6), T(T_M1), -dt, SA, 1, L[L_M],
-                       L[L_M1], L[L_U], L[L_V], 1);
+                       L[L_M1], L[L_U], L[L_V], 1, dif, sd);
   }
   FL_HIP(hipGetLastError());
   std::swap(L[L_M], L[L_M1]);
   // velocity advection -> adjoint of the border-handled state (L_OU, L_OV)
   if ((rc = zero(L_OU)) || (rc = zero(L_OV))) return rc;
+  // faces the passes do not advect keep the viscosity's output: their adjoint (L_SU, L_SV)
+  float* stU = visc ? L[L_SU] : nullptr;
+  float* stV = visc ? L[L_SV] : nullptr;
+  if (visc && ((rc = zero(L_SU)) || (rc = zero(L_SV)))) return rc;
   if (P.use_bfecc) {
     if ((rc = zero(L_CU)) || (rc = zero(L_CV)) || (rc = zero(L_FU)) || (rc = zero(L_FV))) return rc;
     hipLaunchKernelGGL(k_adj_adv_vel, gr, bl, 0, st, g, c->flags, T(T_CU), T(T_CV), T(T_CU), T(T_CV), -dt, L[L_U],
-                       L[L_V], L[L_CU], L[L_CV], L[L_CU], L[L_CV], 1);
+                       L[L_V], L[L_CU], L[L_CV], L[L_CU], L[L_CV], 1, stU, stV);
     hipLaunchKernelGGL(k_adj_corr_vel, gr, bl, 0, st, g, c->flags, T(T_U5), T(T_V5), T(T_FU), T(T_FV), dt, L[L_CU],
                        L[L_CV], L[L_OU], L[L_OV], L[L_FU], L[L_FV]);
     hipLaunchKernelGGL(k_adj_adv_vel, gr, bl, 0, st, g, c->flags, T(T_U5), T(T_V5), T(T_U5), T(T_V5), -dt, L[L_FU],
-                       L[L_FV], L[L_OU], L[L_OV], L[L_OU], L[L_OV], 1);
+                       L[L_FV], L[L_OU], L[L_OV], L[L_OU], L[L_OV], 1, stU, stV);
   } else {
     hipLaunchKernelGGL(k_adj_adv_vel, gr, bl, 0, st, g, c->flags, T(T_U5), T(T_V5), T(T_U5), T(T_V5), -dt, L[L_U],
-                       L[L_V], L[L_OU], L[L_OV], L[L_OU], L[L_OV], 1);
+                       L[L_V], L[L_OU], L[L_OV], L[L_OU], L[L_OV], 1, stU, stV);
   }
   FL_HIP(hipGetLastError());
   const int mr = std::max(c->NX, c->NY);
@@ -863,6 +927,13 @@ static int adj_step(Vjp* J, const float* tp, int K, float dt) {
     hipLaunchKernelGGL(k_mirror_out, gr, bl, 0, st, g, c->S, m->U, m->V, m->p, L[L_OU], L[L_OV]);
     FL_HIP(hipGetLastError());
   }
+  if (visc) {  // L_CU/L_CV are free once the advection's adjoint is done
+    hipLaunchKernelGGL(k_adj_visc, gr, bl, 0, st, g, c->flags, P.viscosity_diffusion * dt, L[L_OU], L[L_OV],
+                       L[L_SU], L[L_SV], L[L_CU], L[L_CV]);
+    FL_HIP(hipGetLastError());
+    std::swap(L[L_OU], L[L_CU]);
+    std::swap(L[L_OV], L[L_CV]);
+  }
   hipLaunchKernelGGL(k_adj_force, gr, bl, 0, st, g, c->flags, L[L_OU], L[L_OV], L[L_GFU], L[L_GFV]);
   if (c->n_src_seg > 0)
     hipLaunchKernelGGL(k_adj_sources, dim3((c->n_src_seg + 255) / 256), dim3(256), 0, st, c->d_src, c->d_src_seg,
@@ -887,8 +958,6 @@ static void vjp_free(Vjp* J) {
 
 static int check_supported(Ctx* c) {
   const fl_params& P = c->prm;
-  if (P.viscosity_diffusion > 0.0f)
-    return verr(FL_ERR_UNSUPPORTED, "the adjoint does not differentiate ViscosityDiffusion > 0");
   if (P.use_multigrid && P.multigrid_levels > 1)
     return verr(FL_ERR_UNSUPPORTED, "the adjoint does not differentiate the multigrid solver");
   if (c->NX < 6 || c->NY < 6) return verr(FL_ERR_UNSUPPORTED, "the adjoint needs at least 4x4 interior cells");

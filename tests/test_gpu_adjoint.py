@@ -14,9 +14,11 @@ from oracle.pyoracle import OracleFluid
 pytestmark = pytest.mark.gpu
 
 
-def jet(make, w, h, iters, bfecc, conf, early=False):
+def jet(make, w, h, iters, bfecc, conf, early=False, visc=0.0, disc=False):
     f = C.jet_scene(make, w, h, iters, jet_u=1.0, jet_rows=max(2, h // 4))
-    f.UseBFECC, f.Confinement, f.EarlyExit = bfecc, conf, early
+    f.UseBFECC, f.Confinement, f.EarlyExit, f.ViscosityDiffusion = bfecc, conf, early, visc
+    if disc:  # more faces the advection leaves alone (they keep the viscosity's output)
+        f.SetCircularObstacle(w // 2, h // 2, 4)
     return f
 
 
@@ -28,12 +30,15 @@ CASES = {
     "sl": dict(bfecc=False, conf=0.0),
     "bfecc": dict(bfecc=True, conf=0.0),
     "bfecc_conf": dict(bfecc=True, conf=0.3),
+    "sl_visc_disc": dict(bfecc=False, conf=0.0, visc=0.15, disc=True),
+    "bfecc_visc": dict(bfecc=True, conf=0.0, visc=0.15),
+    "bfecc_visc_disc": dict(bfecc=True, conf=0.0, visc=0.15, disc=True),
 }
 # Confinement normalises grad|curl| by (|grad|curl|| + 1e-5) (fluid.go:475-477); where |curl|
 # peaks that gradient vanishes and the step's Jacobian reaches ~1e9, so float32 and FP64
 # trajectories have different derivatives there. With confinement the device adjoint is
 # compared with the float32 oracle's adjoint along the identical trajectory only.
-FP64_COMPARABLE = {"sl", "bfecc"}
+FP64_COMPARABLE = {"sl", "bfecc", "sl_visc_disc", "bfecc_visc", "bfecc_visc_disc"}
 
 
 @pytest.mark.parametrize("case", list(CASES), ids=list(CASES))
@@ -115,9 +120,9 @@ def test_early_exit_sweeps_follow_the_forward():
     r.close()
 
 
-def test_rejects_viscosity():
+def test_rejects_multigrid():
     from fluid_amd import FluidError
     g = jet(lambda *a: Fluid(*a), 16, 16, 4, False, 0.0)
-    g.ViscosityDiffusion = 0.1
+    g.UseMultigrid, g.MultigridLevels = True, 2
     with pytest.raises(FluidError):
         Rollout(g, 2)

@@ -365,6 +365,11 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
     off += c->N;
     doff += (size_t)(c->NX + c->NY - 1) * c->NX;
   }
+  {  // item k runs on XCD k % 8: give each XCD a contiguous run of the list
+    std::vector<CellItem> dealt(cells.size());
+    for (size_t k = 0; k < cells.size(); k++) dealt[k] = cells[xcd_tile((int)k, (int)cells.size())];
+    cells.swap(dealt);
+  }
   B->src_gen_seen.assign(n, -1);
   B->ncell_items = (int)cells.size();
   B->nring_items = (int)rings.size();

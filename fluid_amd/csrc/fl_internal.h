@@ -75,6 +75,16 @@ struct RingItem {
   int sim, blk;
 };
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b runs on XCD b % 8), and each
+// XCD has its own L2. xcd_tile(b, B) hands XCD x a contiguous run of the B tiles (row bands of a
+// grid, consecutive items of a work-list), so the neighbour rows a tile's stencils and gathers
+// read were mostly brought into the same L2 by the tiles just before it. A bijection on [0, B):
+// XCD x takes q + (x < B % 8) tiles, starting at x * q + min(x, B % 8), q = B / 8.
+__host__ __device__ inline int xcd_tile(int b, int B) {
+  const int q = B >> 3, rem = B & 7, x = b & 7, r = b >> 3;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
 // Launch maps: one simulation passed by value, or many through a work-list.
 struct OneSim {
   SimDesc d;

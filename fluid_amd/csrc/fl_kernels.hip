@@ -2,7 +2,8 @@
 // solve, the edit/injection kernels and the diagnostic views.
 //
 // Mapping: 64x4 workgroups, threadIdx.x along j (the contiguous axis of the i*NY+j
-// layout) so each wavefront reads and writes 64 consecutive floats of one row.
+// layout) so each wavefront reads and writes 64 consecutive floats of one row; the blocks
+// are dealt to the XCDs in contiguous row bands (xcd_tile).
 // The bilinear gathers hit rows i0, i0+1 at two j's each; at the back-trace lengths
 // of the workloads (<= a few cells) these stay in L1/L2, so the passes run at the
 // stream rate of their field reads and writes. Arithmetic follows the reference
@@ -25,8 +26,11 @@ static inline dim3 grid2d(int NX, int NY) { return dim3((NY + BX - 1) / BX, (NX 
 // ---- launch maps --------------------------------------------------------------------------
 __device__ __forceinline__ bool cell_of(const OneSim& m, const SimDesc*& d, int& i, int& j) {
   d = &m.d;
-  j = blockIdx.x * BX + threadIdx.x;
-  i = blockIdx.y * BY + threadIdx.y;
+  const int gx = gridDim.x;
+  const int t = xcd_tile(blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+  const int by = t / gx, bx = t - by * gx;
+  j = bx * BX + threadIdx.x;
+  i = by * BY + threadIdx.y;
   return i < d->g.NX && j < d->g.NY;
 }
 __device__ __forceinline__ bool cell_of(const ManySims& m, const SimDesc*& d, int& i, int& j) {
@@ -343,6 +347,10 @@ __device__ __forceinline__ float avg_v(const float* __restrict__ V, int c, int n
 // advectVelocity: copyBorder(newU,U), the advected faces, stale newU elsewhere; then U=newU.
 // (Ut,Vt) is the current velocity (traced and sampled); the result goes to newU/newV and
 // to (Uo,Vo).
+// Every load is issued unconditionally, from addresses in range whatever the cell (neighbour
+// reads of cells outside a face's geometric range use a base cell inside the grid; samples of
+// faces that are not advected clamp like any other), so a wave waits for two rounds of loads
+// (the face neighbourhood, then both bilinear samples) instead of one round per branch.
 template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int rUo, int rVo, float dt) {
   const SimDesc* d;
   int i, j;
@@ -355,37 +363,36 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
   float* nV = d->buf[B_NV];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
   const uint32_t f = d->flags[c];
-  int badl = 0;
-  float ou, ov;
-  if (i >= 1 && j >= 1 && j <= NY - 2 && (f & CF_FLUID) && (f & CF_X0)) {
-    float x = (float)i * g.h;
-    float y = (float)j * g.h + g.ha;
-    float u = Ut[c];
-    float v = avg_v(Vt, c, n);
-    x = x - dt * u;
-    y = y - dt * v;
-    ou = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ut, false, true, g, &badl)
-                   : sample_field(x, y, Ut, 0.0f, g.hs, g, &badl);
+  // geometric ranges of the advected faces (u: 1 <= j <= NY-2, v: 1 <= i <= NX-2, both i, j >= 1)
+  const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
+  const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
+  const int cu = gu ? c : n + 1, cv = gv ? c : n + 1;  // (1,1): its neighbourhood is in range
+  const float uc = Ut[c], vc = Vt[c];
+  const float v_at_u = avg_v(Vt, cu, n), u_at_v = avg_u(Ut, cv, n);
+  const float su = ring ? uc : nU[c], sv = ring ? vc : nV[c];  // kept where not advected
+  int badu = 0, badv = 0;
+  float ru, rv;
+  if (g.precise) {
+    ru = sample_cells(i, j, ((-dt) * uc) * g.h1, ((-dt) * v_at_u) * g.h1, Ut, false, true, g, &badu);
+    rv = sample_cells(i, j, ((-dt) * u_at_v) * g.h1, ((-dt) * vc) * g.h1, Vt, true, false, g, &badv);
   } else {
-    ou = ring ? Ut[c] : nU[c];
+    float xu = (float)i * g.h;
+    float yu = (float)j * g.h + g.ha;
+    xu = xu - dt * uc;
+    yu = yu - dt * v_at_u;
+    float xv = (float)i * g.h + g.ha;
+    float yv = (float)j * g.h;
+    xv = xv - dt * u_at_v;
+    yv = yv - dt * vc;
+    ru = sample_field(xu, yu, Ut, 0.0f, g.hs, g, &badu);
+    rv = sample_field(xv, yv, Vt, g.hs, 0.0f, g, &badv);
   }
-  if (j >= 1 && i >= 1 && i <= NX - 2 && (f & CF_FLUID) && (f & CF_Y0)) {
-    float x = (float)i * g.h + g.ha;
-    float y = (float)j * g.h;
-    float u = avg_u(Ut, c, n);
-    float v = Vt[c];
-    x = x - dt * u;
-    y = y - dt * v;
-    ov = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Vt, true, false, g, &badl)
-                   : sample_field(x, y, Vt, g.hs, 0.0f, g, &badl);
-  } else {
-    ov = ring ? Vt[c] : nV[c];
-  }
+  const float ou = au ? ru : su, ov = av ? rv : sv;
   nU[c] = ou;
   d->buf[rUo][c] = ou;
   nV[c] = ov;
   d->buf[rVo][c] = ov;
-  if (badl) atomicOr(d->ctrl + 2, 1);
+  if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
 }
 int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt) {
   const int a = role_of(c, Ut), b = role_of(c, Vt), e = role_of(c, Uo), f = role_of(c, Vo);
@@ -396,24 +403,34 @@ int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, 
   return FL_OK;
 }
 
-// clampToNeighbors (fluid.go:1094-1120) for an interior cell (the 3x3 block is in range).
-__device__ __forceinline__ float clamp_neighbors(float val, const float* __restrict__ src, int c, int n) {
-  float lo = src[c], hi = src[c];
+// clampToNeighbors (fluid.go:1094-1120) for an interior cell (the 3x3 block is in range),
+// in two parts: the 3x3 range of src (taken before the back-trace, whose loads it does not
+// depend on) and the clamp of the corrected value to it. The comparisons are the reference's,
+// so a NaN neighbour is skipped and a NaN centre leaves the value unclamped.
+struct NbRange {
+  float lo, hi;
+};
+__device__ __forceinline__ NbRange neighbor_range(const float* __restrict__ src, int c, int n) {
+  NbRange r{src[c], src[c]};
 #pragma unroll
   for (int di = -1; di <= 1; di++)
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++) {
       float v = src[c + di * n + dj];
-      if (v < lo) lo = v;
-      if (v > hi) hi = v;
+      if (v < r.lo) r.lo = v;
+      if (v > r.hi) r.hi = v;
     }
-  if (val < lo) return lo;
-  if (val > hi) return hi;
+  return r;
+}
+__device__ __forceinline__ float clamp_to(float val, NbRange r) {
+  if (val < r.lo) return r.lo;
+  if (val > r.hi) return r.hi;
   return val;
 }
 
 // advectVelocityBFECC passes 2 and 3 (fluid.go:935-987): backward trace with the
-// original velocity sampling the forward result, then correction and clamp.
+// original velocity sampling the forward result, then correction and clamp. Loads are
+// issued unconditionally from in-range addresses, as in k_advect_velocity.
 template <class M>
 __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, int rUc, int rVc, float dt) {
   const SimDesc* d;
@@ -427,43 +444,37 @@ __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, i
   const float* Vf = d->buf[rVf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
   const uint32_t f = d->flags[c];
-  int badl = 0;
-  float bu, bv;
-  if (i >= 1 && j >= 1 && j <= NY - 2 && (f & CF_FLUID) && (f & CF_X0)) {
-    float x = (float)i * g.h;
-    float y = (float)j * g.h + g.ha;
-    float u = Uo[c];
-    float v = avg_v(Vo, c, n);
-    x = x + dt * u;
-    y = y + dt * v;
-    bu = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Uf, false, true, g, &badl)
-                   : sample_field(x, y, Uf, 0.0f, g.hs, g, &badl);
+  const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
+  const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
+  const int cu = gu ? c : n + 1, cv = gv ? c : n + 1, ci = ring ? n + 1 : c;
+  const float ou = Uo[c], ov = Vo[c];
+  const float v_at_u = avg_v(Vo, cu, n), u_at_v = avg_u(Uo, cv, n);
+  const float fu = Uf[c], fv = Vf[c];
+  const NbRange nu = neighbor_range(Uo, ci, n), nv = neighbor_range(Vo, ci, n);
+  int badu = 0, badv = 0;
+  float ru, rv;
+  if (g.precise) {
+    ru = sample_cells(i, j, (dt * ou) * g.h1, (dt * v_at_u) * g.h1, Uf, false, true, g, &badu);
+    rv = sample_cells(i, j, (dt * u_at_v) * g.h1, (dt * ov) * g.h1, Vf, true, false, g, &badv);
   } else {
-    bu = ring ? Uf[c] : 0.0f;
+    float xu = (float)i * g.h;
+    float yu = (float)j * g.h + g.ha;
+    xu = xu + dt * ou;
+    yu = yu + dt * v_at_u;
+    float xv = (float)i * g.h + g.ha;
+    float yv = (float)j * g.h;
+    xv = xv + dt * u_at_v;
+    yv = yv + dt * ov;
+    ru = sample_field(xu, yu, Uf, 0.0f, g.hs, g, &badu);
+    rv = sample_field(xv, yv, Vf, g.hs, 0.0f, g, &badv);
   }
-  if (j >= 1 && i >= 1 && i <= NX - 2 && (f & CF_FLUID) && (f & CF_Y0)) {
-    float x = (float)i * g.h + g.ha;
-    float y = (float)j * g.h;
-    float u = avg_u(Uo, c, n);
-    float v = Vo[c];
-    x = x + dt * u;
-    y = y + dt * v;
-    bv = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Vf, true, false, g, &badl)
-                   : sample_field(x, y, Vf, g.hs, 0.0f, g, &badl);
-  } else {
-    bv = ring ? Vf[c] : 0.0f;
-  }
-  float ou = Uo[c], ov = Vo[c];
-  float cu = ou, cv = ov;
-  if (f & CF_INTERIOR) {
-    cu = ou - (bu - ou) * 0.5f;
-    cv = ov - (bv - ov) * 0.5f;
-    cu = clamp_neighbors(cu, Uo, c, n);
-    cv = clamp_neighbors(cv, Vo, c, n);
-  }
-  d->buf[rUc][c] = cu;
-  d->buf[rVc][c] = cv;
-  if (badl) atomicOr(d->ctrl + 2, 1);
+  const float bu = au ? ru : (ring ? fu : 0.0f), bv = av ? rv : (ring ? fv : 0.0f);
+  // interior cells: corr = orig - (bwd - orig)/2 clamped to the 3x3 of orig; the ring keeps orig
+  const float kcu = clamp_to(ou - (bu - ou) * 0.5f, nu);
+  const float kcv = clamp_to(ov - (bv - ov) * 0.5f, nv);
+  d->buf[rUc][c] = ring ? ou : kcu;
+  d->buf[rVc][c] = ring ? ov : kcv;
+  if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
 }
 int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd, const float* Vfwd,
                                float* Ucorr, float* Vcorr, float dt) {
@@ -476,7 +487,8 @@ int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, c
   return FL_OK;
 }
 
-// advectSmoke (fluid.go:400-434) using the current (already advected) velocity.
+// advectSmoke (fluid.go:400-434) using the current (already advected) velocity. Loads are
+// issued unconditionally from in-range addresses (ring cells read around cell (1,1)).
 template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, float visc) {
   const SimDesc* d;
   int i, j;
@@ -489,27 +501,27 @@ template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float d
   float* nM = d->buf[B_NM];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
   const uint32_t f = d->flags[c];
-  float out;
-  if ((f & CF_INTERIOR) && (f & CF_FLUID)) {
-    int badl = 0;
-    float u = (U[c] + U[c + n]) * 0.5f * smokeAdv;
-    float v = (V[c] + V[c + 1]) * 0.5f * smokeAdv;
-    float x = (float)i * g.h + g.hs - dt * u;
-    float y = (float)j * g.h + g.hs - dt * v;
-    float val = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ms, true, true, g, &badl)
-                          : sample_field(x, y, Ms, g.hs, g.hs, g, &badl);
-    if (visc > 0.0f) {
-      float sd = visc * 0.3f * dt;
-      float nb = Ms[c - n] + Ms[c + n] + Ms[c - 1] + Ms[c + 1] - 4.0f * Ms[c];
-      val += sd * nb;
-    }
-    out = go_maxf(val, 0.0f);
-    if (badl) atomicOr(d->ctrl + 2, 1);
-  } else {
-    out = ring ? Ms[c] : nM[c];
+  const bool adv = (f & CF_INTERIOR) && (f & CF_FLUID);
+  const int ci = ring ? n + 1 : c;
+  const float mc = Ms[c];
+  const float keep = ring ? mc : nM[c];
+  float u = (U[ci] + U[ci + n]) * 0.5f * smokeAdv;
+  float v = (V[ci] + V[ci + 1]) * 0.5f * smokeAdv;
+  float nb = 0.0f;
+  if (visc > 0.0f) nb = Ms[ci - n] + Ms[ci + n] + Ms[ci - 1] + Ms[ci + 1] - 4.0f * Ms[ci];
+  int badl = 0;
+  float x = (float)i * g.h + g.hs - dt * u;
+  float y = (float)j * g.h + g.hs - dt * v;
+  float val = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ms, true, true, g, &badl)
+                        : sample_field(x, y, Ms, g.hs, g.hs, g, &badl);
+  if (visc > 0.0f) {
+    float sd = visc * 0.3f * dt;
+    val += sd * nb;
   }
+  const float out = adv ? go_maxf(val, 0.0f) : keep;
   nM[c] = out;
   d->buf[rMo][c] = out;
+  if (adv && badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt) {
   const int a = role_of(c, Msrc), b = role_of(c, Mo);
@@ -521,7 +533,8 @@ int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt) {
   return FL_OK;
 }
 
-// advectSmokeBFECC passes 2 and 3 (fluid.go:1013-1046).
+// advectSmokeBFECC passes 2 and 3 (fluid.go:1013-1046). Loads are issued unconditionally
+// from in-range addresses (ring cells read around cell (1,1)).
 template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int rMc, float dt, float smokeAdv) {
   const SimDesc* d;
   int i, j;
@@ -534,25 +547,22 @@ template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int
   const float* Mf = d->buf[rMf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
   const uint32_t f = d->flags[c];
-  float om = Mo[c];
-  float cm = om;
-  if (!ring) {
-    float bm = 0.0f;
-    if (f & CF_FLUID) {
-      int badl = 0;
-      float u = (U[c] + U[c + n]) * 0.5f * smokeAdv;
-      float v = (V[c] + V[c + 1]) * 0.5f * smokeAdv;
-      float x = (float)i * g.h + g.hs + dt * u;
-      float y = (float)j * g.h + g.hs + dt * v;
-      bm = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Mf, true, true, g, &badl)
-                     : sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
-      if (badl) atomicOr(d->ctrl + 2, 1);
-    }
-    cm = om - (bm - om) * 0.5f;
-    cm = clamp_neighbors(cm, Mo, c, n);
-    if (cm < 0.0f) cm = 0.0f;
-  }
-  d->buf[rMc][c] = cm;
+  const bool adv = !ring && (f & CF_FLUID);
+  const int ci = ring ? n + 1 : c;
+  const float om = Mo[c];
+  float u = (U[ci] + U[ci + n]) * 0.5f * smokeAdv;
+  float v = (V[ci] + V[ci + 1]) * 0.5f * smokeAdv;
+  const NbRange nr = neighbor_range(Mo, ci, n);
+  int badl = 0;
+  float x = (float)i * g.h + g.hs + dt * u;
+  float y = (float)j * g.h + g.hs + dt * v;
+  const float bs = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Mf, true, true, g, &badl)
+                             : sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
+  const float bm = adv ? bs : 0.0f;  // solid interior cells: the backward buffer's zero
+  float cm = clamp_to(om - (bm - om) * 0.5f, nr);
+  if (cm < 0.0f) cm = 0.0f;
+  d->buf[rMc][c] = ring ? om : cm;
+  if (adv && badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt) {
   const int a = role_of(c, Morig), b = role_of(c, Mfwd), e = role_of(c, Mcorr);

@@ -195,18 +195,18 @@ static int enqueue_step(Batch* B, float dt) {
   int rc;
   const fl_params& P = B->prm;
   if ((rc = batch_apply_sources(B))) return rc;
-  if ((rc = batch_fill_role(B, B_P, 0.0f))) return rc;
+  // fill(p, 0): the solve reads its initial p from T3 and writes p itself
+  if ((rc = batch_fill_role(B, P.pressure_iters > 0 ? B_T3 : B_P, 0.0f))) return rc;
   if (P.viscosity_diffusion > 0) {
     if ((rc = batch_viscosity(B, dt))) return rc;
     if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
   }
   // The velocity after the solve stays in (U2, V2): confinement, turbulence and borders work
-  // there in place and the (first) velocity advection writes the step's result into (U, V),
-  // so only p and, after semi-Lagrangian smoke, M are copied back.
+  // there in place and the (first) velocity advection writes the step's result into (U, V).
+  // Semi-Lagrangian smoke writes M2, which then trades roles with M (no copy).
   int cu = B_U, cv = B_V;
   if (P.pressure_iters > 0) {
-    if ((rc = batch_gs_solve(B, P.pressure_iters))) return rc;
-    if ((rc = batch_copy_role(B, B_P, B_T3))) return rc;
+    if ((rc = batch_gs_solve(B, P.pressure_iters, B_T3, B_P))) return rc;
     cu = B_U2;
     cv = B_V2;
   }
@@ -225,7 +225,19 @@ static int enqueue_step(Batch* B, float dt) {
   if ((rc = batch_advect_velocity(B, cu, cv, fu, fv, dt))) return rc;
   if (fu != B_U && ((rc = batch_copy_role(B, B_U, fu)) || (rc = batch_copy_role(B, B_V, fv)))) return rc;
   if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
-  return batch_copy_role(B, B_M, B_M2);
+  return batch_swap_roles(B, B_M, B_M2);  // the host side follows in swap_member_roles
+}
+
+// The host half of the step's role swaps: what k_swap_roles did to the device descriptors,
+// applied to the members' pointers and to the copy prepare_batch compares against, so the
+// next step's descriptors match the device's without an upload.
+static void swap_member_roles(Batch* B) {
+  if (B->prm.use_bfecc) return;  // the BFECC smoke's last pass writes M itself
+  for (size_t e = 0; e < B->members.size(); e++) {
+    Ctx* c = B->members[e];
+    std::swap(c->M, c->M2);
+    if (e < B->descs.size()) std::swap(B->descs[e].buf[B_M], B->descs[e].buf[B_M2]);
+  }
 }
 
 static int batch_update(Batch* B, float dt) {
@@ -256,6 +268,7 @@ static int batch_update(Batch* B, float dt) {
   } else {
     if ((rc = enqueue_step(B, dt))) return rc;
   }
+  swap_member_roles(B);
   for (Ctx* c : B->members) {
     c->last_n = B->prm.pressure_iters;
     c->host_sweeps = -1;
@@ -452,7 +465,8 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
   for (int tk : {8, 16}) {
     B->prm.block_sweeps = tk;
     if ((rc = prepare_batch(B, dt))) return rc;
-    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters); }, 2, &rc);
+    // trial solves write only scratch roles (U2, V2, T2), so the members' state is unchanged
+    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters, B_T3, B_T2); }, 2, &rc);
     if (rc) return rc;
     if (t < best_ms) {
       best_ms = t;

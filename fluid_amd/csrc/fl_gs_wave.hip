@@ -50,6 +50,7 @@ struct GsRun {
   int32_t* ticket;
   const float* omega;
   float damping;
+  int rPin, rPout;    // roles of the initial p (read) and of the solve's p (written)
 };
 
 __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX) {
@@ -127,10 +128,10 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     }
     const float* Uin = D.buf[B_U];
     const float* Vin = D.buf[B_V];
-    const float* Pin = D.buf[B_P];
+    const float* Pin = D.buf[R.rPin];
     float* Uout = D.buf[B_U2];
     float* Vout = D.buf[B_V2];
-    float* Pout = D.buf[B_T3];
+    float* Pout = D.buf[R.rPout];
     const uint8_t* fdiag = D.diag;
     float* kout = D.kout;
     float* gout = D.gout;
@@ -398,12 +399,12 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
 
 // Faces and cells no interior cell touches keep their input values: U with i = 0 or
 // j in {0, NY-1}; V with i in {0, NX-1} or j = 0; p on the ring. All lie on the ring.
-__device__ __forceinline__ void ring_copy(const SimDesc& D, int i, int j) {
+__device__ __forceinline__ void ring_copy(const SimDesc& D, int i, int j, int rPin, int rPout) {
   const int NX = D.g.NX, NY = D.g.NY;
   const size_t c = (size_t)i * NY + j;
   if (!(i >= 1 && j >= 1 && j <= NY - 2)) D.buf[B_U2][c] = D.buf[B_U][c];
   if (!(i >= 1 && i <= NX - 2 && j >= 1)) D.buf[B_V2][c] = D.buf[B_V][c];
-  D.buf[B_T3][c] = D.buf[B_P][c];
+  D.buf[rPout][c] = D.buf[rPin][c];
 }
 __device__ __forceinline__ void ring_desc(const OneSim& m, const SimDesc*& D, int& t) {
   D = &m.d;
@@ -414,18 +415,18 @@ __device__ __forceinline__ void ring_desc(const ManySims& m, const SimDesc*& D, 
   D = &m.sims[it.sim];
   t = it.blk * blockDim.x + threadIdx.x;
 }
-template <class M> __global__ void k_gs_ring(M m) {
+template <class M> __global__ void k_gs_ring(M m, int rPin, int rPout) {
   const SimDesc* D;
   int t;
   ring_desc(m, D, t);
   const int NX = D->g.NX, NY = D->g.NY;
   if (t < NX) {
-    ring_copy(*D, t, 0);
-    ring_copy(*D, t, NY - 1);
+    ring_copy(*D, t, 0, rPin, rPout);
+    ring_copy(*D, t, NY - 1, rPin, rPout);
   }
   if (t < NY) {
-    ring_copy(*D, 0, t);
-    ring_copy(*D, NX - 1, t);
+    ring_copy(*D, 0, t, rPin, rPout);
+    ring_copy(*D, NX - 1, t, rPin, rPout);
   }
 }
 
@@ -553,7 +554,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   OneSim m{d};
   const int nprog = W.G * W.KB;
   const int mr = std::max(c->NX, c->NY);
-  hipLaunchKernelGGL(k_gs_ring<OneSim>, dim3((mr + 255) / 256), dim3(256), 0, c->stream, m);
+  hipLaunchKernelGGL(k_gs_ring<OneSim>, dim3((mr + 255) / 256), dim3(256), 0, c->stream, m, (int)B_P, (int)B_T3);
   hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, 2 * nprog, c->d_maxdiv, 2 * c->nmax,
                      c->d_ctrl, 1, W.d_ticket);
   FL_HIP(hipGetLastError());
@@ -563,6 +564,8 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.K = N;
   R.omega = c->d_omega;
   R.damping = c->prm.pressure_damping;
+  R.rPin = B_P;  // then p and T3 trade places
+  R.rPout = B_T3;
   for (int which = 0; which < 2; which++) {
     R.which = which;
     R.ticket = W.d_ticket + which;
@@ -596,7 +599,8 @@ int pressure_free(Ctx* c) {
 
 // ---- batched solve ------------------------------------------------------------------------
 // All members' tiles in one ticket list, merged by hyperplane start; one persistent launch
-// drains it (and one more for the members that early-exit). Results land in U2, V2, T3.
+// drains it (and one more for the members that early-exit). The initial p is read from role
+// rPin, the solve's p written to rPout, the velocity to U2, V2.
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   long long t16 = 0;
   for (const SimDesc& d : descs) t16 += tiles_at16(d.g.NX, N);
@@ -670,10 +674,10 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   return FL_OK;
 }
 
-int batch_gs_solve(Batch* B, int N) {
+int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
   ManySims m{B->d_sims, B->d_cells, B->d_rings};
   const int E = (int)B->members.size();
-  hipLaunchKernelGGL(k_gs_ring<ManySims>, dim3(B->nring_items), dim3(256), 0, B->stream, m);
+  hipLaunchKernelGGL(k_gs_ring<ManySims>, dim3(B->nring_items), dim3(256), 0, B->stream, m, rPin, rPout);
   // progress counters, per-sweep maxima and re-run requests of all members are contiguous
   hipLaunchKernelGGL(k_gs_init, dim3(256), dim3(256), 0, B->stream, B->prog, B->gs_nprog, B->maxdiv_pool,
                      2 * B->nmax * E, B->ctrl_pool, E, B->ticket);
@@ -684,6 +688,8 @@ int batch_gs_solve(Batch* B, int N) {
   R.K = N;
   R.omega = B->d_omega;
   R.damping = B->prm.pressure_damping;
+  R.rPin = rPin;
+  R.rPout = rPout;
   for (int which = 0; which < 2; which++) {
     R.which = which;
     R.ticket = B->ticket + which;

@@ -299,6 +299,7 @@ int launch_sample_batch(Ctx* c, const float* xy, int n, float* uv);
 // ---- batched passes (fl_kernels.hip): one launch over all members of a batch -----------
 int batch_fill_role(Batch* B, int r, float v);
 int batch_copy_role(Batch* B, int rdst, int rsrc);
+int batch_swap_roles(Batch* B, int a, int b);  // in every member's device descriptor
 int batch_flags(Batch* B);
 int batch_viscosity(Batch* B, float dt);
 int batch_conf_turb(Batch* B, float dt, int rU, int rV);
@@ -313,7 +314,7 @@ int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const i
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
 int pick_tk(const fl_params& P, int N, long long tiles16);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
-int batch_gs_solve(Batch* B, int N);
+int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p read from rPin, written to rPout
 
 // ---- pressure (fl_pressure.hip) -------------------------------------------------------
 // Runs the single-grid projection: sweeps with the reference order on (U,V,p).

@@ -589,6 +589,22 @@ int batch_copy_role(Batch* B, int rdst, int rsrc) {
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
+// Swap two roles in every member's descriptor: the ping-pong of a field whose pass writes
+// its partner (semi-Lagrangian smoke writes M2), in place of copying it back. The host swaps
+// the members' pointers to match after enqueueing the step (swap_member_roles).
+__global__ void k_swap_roles(SimDesc* sims, int n, int a, int b) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float* t = sims[e].buf[a];
+  sims[e].buf[a] = sims[e].buf[b];
+  sims[e].buf[b] = t;
+}
+int batch_swap_roles(Batch* B, int a, int b) {
+  const int n = (int)B->members.size();
+  hipLaunchKernelGGL(k_swap_roles, dim3((n + 63) / 64), dim3(64), 0, B->stream, B->d_sims, n, a, b);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
+}
 int batch_flags(Batch* B) {
   hipLaunchKernelGGL(k_flags<ManySims>, FL_BATCH_CELLS(B), many(B));
   FL_HIP(hipGetLastError());

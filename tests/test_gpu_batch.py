@@ -113,11 +113,13 @@ def test_batch_member_vs_oracle():
     b.close()
 
 
-def test_graph_replays_track_changes():
+@pytest.mark.parametrize("kset", ["bfecc_conf", "sl_visc"])
+def test_graph_replays_track_changes(kset):
     """Replays of the captured step stay identical to solo stepping while edits, knob
-    changes, new sources and a member's own Simulate happen between them."""
+    changes, new sources and a member's own Simulate happen between them. With
+    semi-Lagrangian smoke every step trades the M and M2 roles of all members."""
     sizes = SIZES[:5]
-    knobs = KNOB_SETS["bfecc_conf"]
+    knobs = KNOB_SETS[kset]
     b, solo = make_pair(sizes, knobs)
 
     def both(fn):

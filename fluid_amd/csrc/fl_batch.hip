@@ -195,8 +195,8 @@ static int enqueue_step(Batch* B, float dt) {
   int rc;
   const fl_params& P = B->prm;
   if ((rc = batch_apply_sources(B))) return rc;
-  // fill(p, 0): the solve reads its initial p from T3 and writes p itself
-  if ((rc = batch_fill_role(B, P.pressure_iters > 0 ? B_T3 : B_P, 0.0f))) return rc;
+  // fill(p, 0): the solve starts from p = 0 without reading it and writes p itself
+  if (P.pressure_iters <= 0 && (rc = batch_fill_role(B, B_P, 0.0f))) return rc;
   if (P.viscosity_diffusion > 0) {
     if ((rc = batch_viscosity(B, dt))) return rc;
     if ((rc = batch_copy_role(B, B_U, B_U2)) || (rc = batch_copy_role(B, B_V, B_V2))) return rc;
@@ -206,7 +206,7 @@ static int enqueue_step(Batch* B, float dt) {
   // Semi-Lagrangian smoke writes M2, which then trades roles with M (no copy).
   int cu = B_U, cv = B_V;
   if (P.pressure_iters > 0) {
-    if ((rc = batch_gs_solve(B, P.pressure_iters, B_T3, B_P))) return rc;
+    if ((rc = batch_gs_solve(B, P.pressure_iters, -1, B_P))) return rc;
     cu = B_U2;
     cv = B_V2;
   }
@@ -466,7 +466,7 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
     B->prm.block_sweeps = tk;
     if ((rc = prepare_batch(B, dt))) return rc;
     // trial solves write only scratch roles (U2, V2, T2), so the members' state is unchanged
-    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters, B_T3, B_T2); }, 2, &rc);
+    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters, -1, B_T2); }, 2, &rc);
     if (rc) return rc;
     if (t < best_ms) {
       best_ms = t;

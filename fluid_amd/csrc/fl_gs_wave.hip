@@ -50,7 +50,7 @@ struct GsRun {
   int32_t* ticket;
   const float* omega;
   float damping;
-  int rPin, rPout;    // roles of the initial p (read) and of the solve's p (written)
+  int rPin, rPout;    // roles of the initial p (read; < 0: p starts at zero) and of the solve's p
 };
 
 __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX) {
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     }
     const float* Uin = D.buf[B_U];
     const float* Vin = D.buf[B_V];
-    const float* Pin = D.buf[R.rPin];
+    const float* Pin = R.rPin >= 0 ? D.buf[R.rPin] : nullptr;
     float* Uout = D.buf[B_U2];
     float* Vout = D.buf[B_V2];
     float* Pout = D.buf[R.rPout];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
               if (r + 1 >= 0 && r + 1 <= NX - 1) v.x = Uin[(size_t)(r + 1) * NY + j];
               if (r >= 0 && r <= NX - 1 && j + 1 <= NY - 1) {
                 v.y = Vin[(size_t)r * NY + j + 1];
-                v.z = Pin[(size_t)r * NY + j + 1];
+                v.z = Pin ? Pin[(size_t)r * NY + j + 1] : 0.0f;
               }
             }
           } else {
@@ -404,7 +404,7 @@ __device__ __forceinline__ void ring_copy(const SimDesc& D, int i, int j, int rP
   const size_t c = (size_t)i * NY + j;
   if (!(i >= 1 && j >= 1 && j <= NY - 2)) D.buf[B_U2][c] = D.buf[B_U][c];
   if (!(i >= 1 && i <= NX - 2 && j >= 1)) D.buf[B_V2][c] = D.buf[B_V][c];
-  D.buf[rPout][c] = D.buf[rPin][c];
+  D.buf[rPout][c] = rPin >= 0 ? D.buf[rPin][c] : 0.0f;
 }
 __device__ __forceinline__ void ring_desc(const OneSim& m, const SimDesc*& D, int& t) {
   D = &m.d;
@@ -600,7 +600,8 @@ int pressure_free(Ctx* c) {
 // ---- batched solve ------------------------------------------------------------------------
 // All members' tiles in one ticket list, merged by hyperplane start; one persistent launch
 // drains it (and one more for the members that early-exit). The initial p is read from role
-// rPin, the solve's p written to rPout, the velocity to U2, V2.
+// rPin (rPin < 0: all zero, Simulate's fill(p, 0)), the solve's p written to rPout, the
+// velocity to U2, V2.
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   long long t16 = 0;
   for (const SimDesc& d : descs) t16 += tiles_at16(d.g.NX, N);

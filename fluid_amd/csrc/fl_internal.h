@@ -314,7 +314,7 @@ int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const i
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
 int pick_tk(const fl_params& P, int N, long long tiles16);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
-int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p read from rPin, written to rPout
+int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p from rPin (< 0: zero), to rPout
 
 // ---- pressure (fl_pressure.hip) -------------------------------------------------------
 // Runs the single-grid projection: sweeps with the reference order on (U,V,p).

@@ -260,18 +260,24 @@ __global__ void k_conf_turb(M m, int rU, int rV, int rcurl, float conf, float dt
     u += strength * gy * vort * dt;
     v -= strength * gx * vort * dt;
   }
+  // the update is in place: a face the passes leave alone is not written back, and the
+  // noise (two float64 sines) is evaluated only where the speed gate lets it in
+  bool changed = do_conf != 0;
   if (do_turb) {
-    float noiseU = turb_noise(i * 137 + j * 241) * turb;
-    float noiseV = turb_noise(i * 157 + j * 263) * turb;
     float localVel = sqrtf(u * u + v * v);
     if (localVel > 0.1f) {
+      float noiseU = turb_noise(i * 137 + j * 241) * turb;
+      float noiseV = turb_noise(i * 157 + j * 263) * turb;
       float factor = go_minf(localVel * 0.5f, 1.0f);
       u += noiseU * factor;
       v += noiseV * factor;
+      changed = true;
     }
   }
-  U[c] = u;
-  V[c] = v;
+  if (changed) {
+    U[c] = u;
+    V[c] = v;
+  }
 }
 int launch_conf_turb(Ctx* c, const float* curl, float dt, int do_conf, int do_turb) {
   const float turb = c->prm.turbulence_strength * dt, conf = c->prm.confinement;

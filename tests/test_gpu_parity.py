@@ -8,9 +8,20 @@ import pytest
 
 from fluid_amd import Fluid
 from fluid_amd import configs as C
+from known_answers import load_cases, run_case
 from oracle.pyoracle import OracleFluid
 
 pytestmark = pytest.mark.gpu
+
+KNOWN = load_cases()
+
+
+@pytest.mark.parametrize("case", KNOWN, ids=[c["name"] for c in KNOWN])
+def test_reference_known_answers_device(case):
+    """The reference tests' known answers (tests/golden/reference_known_answers.json) on the
+    device library, through the same runner the oracle's CPU test uses."""
+    n, fails = run_case(lambda *a: Fluid(*a), case)
+    assert n > 0 and not fails, fails
 
 FIELDS = ("U", "V", "newU", "newV", "p", "S", "M", "newM")
 

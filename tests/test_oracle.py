@@ -5,12 +5,14 @@ sign, a wrong face index, a transposed operand) fails it: the reference's own
 known-answer tests (restated from pkg/fluid/*_test.go), closed forms, invariants of
 the projection, and agreement of the float32 and float64 instantiations.
 """
+import copy
 import math
 import struct
 
 import numpy as np
 import pytest
 
+from known_answers import load_cases, run_case
 from oracle.pyoracle import OracleFluid, go_sin, go_sin_many, noise_table
 
 F32 = np.float32
@@ -63,6 +65,29 @@ def test_turbulence_noise_table_is_implementation_independent():
 
 
 # ---- restated reference tests with exact answers ---------------------------------------------
+
+KNOWN = load_cases()
+
+
+@pytest.mark.parametrize("case", KNOWN, ids=[c["name"] for c in KNOWN])
+def test_reference_known_answers(case):
+    """The fixture tests/golden/reference_known_answers.json: each reference test's own
+    assertions plus the closed forms of the same state."""
+    n, fails = run_case(lambda *a: OracleFluid(*a), case)
+    assert n > 0 and not fails, fails
+
+
+def test_known_answer_runner_reports_a_wrong_answer():
+    """The runner is not vacuous: a case whose expected value is off by one float32 ulp, or
+    whose expected view value has the wrong sign, is reported."""
+    case = copy.deepcopy(next(c for c in KNOWN if c["name"] == "TestApplyForceRadius"))
+    chk = case["exact"][0]
+    chk[5] = float(np.nextafter(F32(chk[5]), F32(1e9)))
+    assert len(run_case(lambda *a: OracleFluid(*a), case)[1]) == 1
+    case = copy.deepcopy(next(c for c in KNOWN if c["name"] == "TestVorticityField"))
+    case["exact"][0][5] = -2.0
+    assert len(run_case(lambda *a: OracleFluid(*a), case)[1]) == 1
+
 
 def test_copy_border_exact():
     # TestCopyBorder (fluid_test.go:8-35)

@@ -177,11 +177,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     constexpr int NITEM = BLK * (LANES + 1) + BLK * TK;
     constexpr int PER = (NITEM + NT - 1) / NT;
     float4 stage[PER];
+    float2 corner[PER];  // K-stream lane 0: V and p of the corner producer, merged in store_block
     auto load_block = [&](int blk) {
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int it = tid + q * NT;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 cz = make_float2(0.f, 0.f);
         if (it < BLK * (LANES + 1)) {
           const int u = it / (LANES + 1), slot = it % (LANES + 1);
           const int s = blk * BLK + u;
@@ -200,16 +202,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
               const int rr = min(s + TK - 1, rows - 1);
               v = ld_dev3(kout + ((size_t)pK * rows + rr) * 3 * (LANES + 1) + slot, LANES + 1);
             }
-            if (slot == 0) {
-              float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-              if (pC >= 0) {
-                const int rr = min(s + 63 + TK, rows - 1);
-                const float* cpp = kout + ((size_t)pC * rows + rr) * 3 * (LANES + 1) + LANES;
-                c.y = ld_dev(cpp + (LANES + 1));
-                c.z = ld_dev(cpp + 2 * (LANES + 1));
-              }
-              v.y = c.y;
-              v.z = c.z;
+            // lane 0 takes V and p from the corner producer instead (zero without one); the
+            // values are merged in store_block, so no wait for these loads happens here
+            if (slot == 0 && pC >= 0) {
+              const int rr = min(s + 63 + TK, rows - 1);
+              const float* cpp = kout + ((size_t)pC * rows + rr) * 3 * (LANES + 1) + LANES;
+              cz.x = ld_dev(cpp + (LANES + 1));
+              cz.y = ld_dev(cpp + 2 * (LANES + 1));
             }
           }
         } else if (it < NITEM) {
@@ -222,6 +221,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
           }
         }
         stage[q] = v;
+        corner[q] = cz;
       }
     };
     auto store_block = [&](int buf) {
@@ -229,7 +229,11 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
       for (int q = 0; q < PER; q++) {
         const int it = tid + q * NT;
         if (it < BLK * (LANES + 1)) {
-          Kst[buf][it / (LANES + 1)][it % (LANES + 1)] = stage[q];
+          float4 v = stage[q];
+          const bool lane0 = kb > 0 && it % (LANES + 1) == 0;  // selects read both sides on every path
+          v.y = lane0 ? corner[q].x : v.y;
+          v.z = lane0 ? corner[q].y : v.z;
+          Kst[buf][it / (LANES + 1)][it % (LANES + 1)] = v;
         } else if (it < NITEM) {
           const int e = it - BLK * (LANES + 1);
           Gst[buf][e / TK][e % TK] = stage[q];

@@ -174,7 +174,12 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     };
 
     // Staged boundary items of one block: K stream 8x65, G stream 8xTK.
-    constexpr int NITEM = BLK * (LANES + 1) + BLK * TK;
+    // Items [0, KITEM) are the K stream; the G stream starts at the next wave boundary, so no
+    // wave holds lanes of both (a wave running both branches would wait, in the second, for
+    // the loads the first issued into the registers both use).
+    constexpr int KITEM = BLK * (LANES + 1);
+    constexpr int GITEM0 = (KITEM + LANES - 1) / LANES * LANES;
+    constexpr int NITEM = GITEM0 + BLK * TK;
     constexpr int PER = (NITEM + NT - 1) / NT;
     float4 stage[PER];
     float2 corner[PER];  // K-stream lane 0: V and p of the corner producer, merged in store_block
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         const int it = tid + q * NT;
         float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
         float2 cz = make_float2(0.f, 0.f);
-        if (it < BLK * (LANES + 1)) {
+        if (it < KITEM) {
           const int u = it / (LANES + 1), slot = it % (LANES + 1);
           const int s = blk * BLK + u;
           if (kb == 0) {
@@ -211,8 +216,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
               cz.y = ld_dev(cpp + 2 * (LANES + 1));
             }
           }
-        } else if (it < NITEM) {
-          const int e = it - BLK * (LANES + 1);
+        } else if (it >= GITEM0 && it < NITEM) {
+          const int e = it - GITEM0;
           const int u = e / TK, b = e % TK;
           const int s = blk * BLK + u;
           if (pG >= 0) {
@@ -228,14 +233,14 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
 #pragma unroll
       for (int q = 0; q < PER; q++) {
         const int it = tid + q * NT;
-        if (it < BLK * (LANES + 1)) {
+        if (it < KITEM) {
           float4 v = stage[q];
           const bool lane0 = kb > 0 && it % (LANES + 1) == 0;  // selects read both sides on every path
           v.y = lane0 ? corner[q].x : v.y;
           v.z = lane0 ? corner[q].y : v.z;
           Kst[buf][it / (LANES + 1)][it % (LANES + 1)] = v;
-        } else if (it < NITEM) {
-          const int e = it - BLK * (LANES + 1);
+        } else if (it >= GITEM0 && it < NITEM) {
+          const int e = it - GITEM0;
           Gst[buf][e / TK][e % TK] = stage[q];
         }
       }

@@ -30,6 +30,7 @@ class FlParams(ctypes.Structure):
         ("solver", ctypes.c_int32),
         ("block_sweeps", ctypes.c_int32),
         ("precise_advection", ctypes.c_int32),
+        ("handoff_lag", ctypes.c_int32),
     ]
 
 

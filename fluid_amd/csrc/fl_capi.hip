@@ -297,6 +297,7 @@ fl_params default_params() {
   p.solver = 0;
   p.block_sweeps = 0;
   p.precise_advection = 0;
+  p.handoff_lag = 0;
   return p;
 }
 

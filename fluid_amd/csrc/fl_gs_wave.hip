@@ -51,6 +51,7 @@ struct GsRun {
   const float* omega;
   float damping;
   int rPin, rPout;    // roles of the initial p (read; < 0: p starts at zero) and of the solve's p
+  int defer;          // publish handoff rows one block late (publish_rows below)
 };
 
 __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX) {
@@ -74,6 +75,13 @@ __device__ __forceinline__ float4 ld_dev3(const float* p, int stride) {
 }
 // Wait until every store this wave issued has been acknowledged by the memory system.
 __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Wait until at most N of this wave's vector memory operations are in flight. The counter
+// retires loads and stores in issue order, so every operation issued before the N most
+// recent ones is then complete.
+template <int N> __device__ __forceinline__ void wait_stores_but() {
+  static_assert(N >= 0 && N <= 63, "vmcnt is a 6-bit count");
+  asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
 
 __device__ __forceinline__ int wait_progress(int32_t* p, int need, int32_t* err) {
   int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -256,8 +264,20 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         fl[u] = f;
       }
     };
-    // Producer side: copy the G stream of block blk to global, then publish.
-    auto publish = [&](int blk) {
+    // Producer side, in two halves run by the last wave (which also stores the K-stream
+    // rows, three stores per step). publish_data(blk) copies the G stream of block blk to
+    // global without waiting. publish_rows(n) makes handoff rows [0, n) visible: it waits
+    // for the wave's stores of those rows, then stores the progress counter. On time, the
+    // rows of block blk are published at the start of block blk + 1 after a full wait. With
+    // R.defer they are published at the start of block blk + 2, right after
+    // publish_data(blk + 1): by then the wave has issued at least NEWER stores after them
+    // (the 3 * BLK K-stream stores of block blk + 1 and the G copy of block blk + 1), and
+    // stores retire in issue order, so waiting until at most NEWER are in flight completes
+    // them without waiting for the stores just issued. Deferring removes a store round trip
+    // from every block and lengthens the pipeline fill by a block per producer hop. After
+    // the loop the last blocks are published with a full wait.
+    constexpr int NEWER = 3 * BLK + 3 * ((BLK * TK + LANES - 1) / LANES);
+    auto publish_data = [&](int blk) {
       if (wave == TK - 1) {
         for (int e = lane; e < BLK * TK; e += LANES) {
           const int u = e / TK, b = e % TK;
@@ -267,8 +287,15 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
           st_dev(dst + TK, v.y);
           st_dev(dst + 2 * TK, v.z);
         }
-        wait_stores();
-        if (lane == 0) __hip_atomic_store(&prog[me], (blk + 1) * BLK, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    };
+    auto publish_rows = [&](int n, bool all) {
+      if (wave == TK - 1) {
+        if (all)
+          wait_stores();
+        else
+          wait_stores_but<NEWER>();
+        if (lane == 0) __hip_atomic_store(&prog[me], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     };
 
@@ -284,7 +311,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     float UrOut = 0.f, Vreg = 0.f, pn = 0.f, md = 0.f;
     for (int blk = 0; blk < nblk; blk++) {
       const int cb = blk & 1;
-      if (blk > 0) publish(blk - 1);
+      if (blk > 0) {
+        publish_data(blk - 1);
+        if (!R.defer)
+          publish_rows(blk * BLK, true);
+        else if (blk > 1)
+          publish_rows((blk - 1) * BLK, false);
+      }
       const bool more = blk + 1 < nblk;
       if (more) {
         wait_block(blk + 1);
@@ -397,7 +430,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         __syncthreads();
       }
     }
-    publish(nblk - 1);
+    publish_data(nblk - 1);
+    publish_rows(rows, true);  // rows = nblk * BLK
     // per-sweep max |div|
     float mv = md;
     for (int o = 32; o > 0; o >>= 1) mv = fmaxf(mv, __shfl_xor(mv, o, 64));
@@ -507,6 +541,13 @@ int pick_tk(const fl_params& P, int N, long long tiles16) {
   if (P.block_sweeps == 8 || P.block_sweeps == 16) return P.block_sweeps;
   return (N >= 64 && tiles16 >= 512) ? 16 : 8;
 }
+// Publish the handoff rows one block late (handoff_lag 2), on time (1), or (0) late when the
+// tiles fill the persistent grid at least twice: then the solve is throughput-bound and the
+// per-block store wait costs more than the longer pipeline fill.
+static int defer_handoff(const fl_params& P, int ntiles, int grid) {
+  if (P.handoff_lag == 1 || P.handoff_lag == 2) return P.handoff_lag == 2;
+  return ntiles >= 2 * grid ? 1 : 0;
+}
 static long long tiles_at16(int NX, int N) { return (long long)((NX + N - 1 + 63) / 64) * ((N + 15) / 16); }
 
 template <int TK> static int occupancy_grid(int dev, int ntiles, bool many) {
@@ -575,6 +616,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.damping = c->prm.pressure_damping;
   R.rPin = B_P;  // then p and T3 trade places
   R.rPout = B_T3;
+  R.defer = defer_handoff(c->prm, W.ntiles, W.grid);
   for (int which = 0; which < 2; which++) {
     R.which = which;
     R.ticket = W.d_ticket + which;
@@ -700,6 +742,7 @@ int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
   R.damping = B->prm.pressure_damping;
   R.rPin = rPin;
   R.rPout = rPout;
+  R.defer = defer_handoff(B->prm, B->ntiles, B->grid);
   for (int which = 0; which < 2; which++) {
     R.which = which;
     R.ticket = B->ticket + which;

@@ -86,6 +86,7 @@ def _init_knobs(o):
     # Addition: back-trace in cell units (no reference counterpart; off = the reference's
     # absolute-position arithmetic, bit for bit).
     o.PreciseAdvection = False
+    o.HandoffLag = 0     # temporal-blocked solve: 0 auto, 1 on time, 2 one block late
 
 
 def _params_of(o):
@@ -104,6 +105,7 @@ def _params_of(o):
     p.solver = int(o.Solver)
     p.block_sweeps = int(o.BlockSweeps)
     p.precise_advection = int(bool(o.PreciseAdvection))
+    p.handoff_lag = int(o.HandoffLag)
     return p
 
 

@@ -70,6 +70,9 @@ typedef struct fl_params {
   int32_t solver;            /* 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel */
   int32_t block_sweeps;      /* sweeps per temporal block (0 = tuned default) */
   int32_t precise_advection; /* 1: back-trace in cell units (no reference counterpart), default 0 */
+  int32_t handoff_lag;       /* temporal-blocked solve: 0 auto, 1 publish each block's handoff rows
+                                at the next block's start, 2 one block later (fewer waits, longer
+                                pipeline fill; auto picks 2 from two rounds of tiles up) */
 } fl_params;
 
 /* Statistics of the last Simulate (no reference counterpart). */

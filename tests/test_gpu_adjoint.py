@@ -14,9 +14,11 @@ from oracle.pyoracle import OracleFluid
 pytestmark = pytest.mark.gpu
 
 
-def jet(make, w, h, iters, bfecc, conf, early=False, visc=0.0, disc=False):
+def jet(make, w, h, iters, bfecc, conf, early=False, visc=0.0, disc=False, lag=0):
     f = C.jet_scene(make, w, h, iters, jet_u=1.0, jet_rows=max(2, h // 4))
     f.UseBFECC, f.Confinement, f.EarlyExit, f.ViscosityDiffusion = bfecc, conf, early, visc
+    if lag:  # the device's handoff publication (forward, recompute and mirrored solves)
+        f.HandoffLag = lag
     if disc:  # more faces the advection leaves alone (they keep the viscosity's output)
         f.SetCircularObstacle(w // 2, h // 2, 4)
     return f
@@ -33,12 +35,13 @@ CASES = {
     "sl_visc_disc": dict(bfecc=False, conf=0.0, visc=0.15, disc=True),
     "bfecc_visc": dict(bfecc=True, conf=0.0, visc=0.15),
     "bfecc_visc_disc": dict(bfecc=True, conf=0.0, visc=0.15, disc=True),
+    "bfecc_late_handoff": dict(bfecc=True, conf=0.0, lag=2),
 }
 # Confinement normalises grad|curl| by (|grad|curl|| + 1e-5) (fluid.go:475-477); where |curl|
 # peaks that gradient vanishes and the step's Jacobian reaches ~1e9, so float32 and FP64
 # trajectories have different derivatives there. With confinement the device adjoint is
 # compared with the float32 oracle's adjoint along the identical trajectory only.
-FP64_COMPARABLE = {"sl", "bfecc", "sl_visc_disc", "bfecc_visc", "bfecc_visc_disc"}
+FP64_COMPARABLE = {"sl", "bfecc", "sl_visc_disc", "bfecc_visc", "bfecc_visc_disc", "bfecc_late_handoff"}
 
 
 @pytest.mark.parametrize("case", list(CASES), ids=list(CASES))

@@ -47,7 +47,8 @@ def scene(e, f):
 
 
 BASE_KNOBS = dict(Confinement=0.0, ViscosityDiffusion=0.0, PressureDamping=1.0, TurbulenceStrength=0.02,
-                  SmokeAdvection=1.0, UseBFECC=False, PressureIters=8, EarlyExit=True, BlockSweeps=0)
+                  SmokeAdvection=1.0, UseBFECC=False, PressureIters=8, EarlyExit=True, BlockSweeps=0,
+                  HandoffLag=0)
 
 
 def set_knobs(dst, knobs):
@@ -60,6 +61,7 @@ KNOB_SETS = {
     "bfecc_conf": dict(UseBFECC=True, Confinement=0.3, PressureIters=40, EarlyExit=True),
     "sl_visc": dict(UseBFECC=False, ViscosityDiffusion=0.002, PressureIters=24, EarlyExit=False),
     "tk8_damped": dict(UseBFECC=True, PressureDamping=0.8, PressureIters=20, EarlyExit=True, BlockSweeps=8),
+    "late_handoff": dict(UseBFECC=True, PressureIters=40, EarlyExit=True, HandoffLag=2),
 }
 
 

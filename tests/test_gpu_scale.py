@@ -25,7 +25,8 @@ def same(g, o, where, fields=FIELDS):
 
 @pytest.mark.parametrize("tk", [8, 16])
 @pytest.mark.parametrize("early", [False, True])
-def test_many_tiles_bit_exact(tk, early):
+@pytest.mark.parametrize("lag", [1, 2], ids=["on_time", "one_block_late"])
+def test_many_tiles_bit_exact(tk, early, lag):
     w, h, n = 700, 300, 100
     rng = np.random.default_rng(21)
     st = {}
@@ -42,6 +43,7 @@ def test_many_tiles_bit_exact(tk, early):
         f.EarlyExit = early
         f.UseBFECC = True
     g.BlockSweeps = tk
+    g.HandoffLag = lag  # both ways of publishing the tile handoffs
     for step in range(3):
         g.Simulate(0.1)
         o.Simulate(0.1)

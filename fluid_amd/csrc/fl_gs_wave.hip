@@ -48,6 +48,7 @@ struct GsRun {
   int which;          // 0 main run (K sweeps), 1 re-run (ctrl[1] sweeps of each simulation)
   int K;
   int32_t* ticket;
+  const int32_t* rerun;  // re-run launch only: nonzero when some simulation asked for one
   const float* omega;
   float damping;
   int rPin, rPout;    // roles of the initial p (read; < 0: p starts at zero) and of the solve's p
@@ -114,6 +115,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
   __shared__ float4 Gos[2][BLK][TK];
   __shared__ int s_tile;
   __shared__ int s_seen[3];
+  // a re-run launch with nothing to re-run ends here, in every workgroup alike
+  if (R.rerun && *R.rerun == 0) return;
 
   for (;;) {
     if (tid == 0) s_tile = atomicAdd(R.ticket, 1);
@@ -482,26 +485,34 @@ __global__ void k_gs_init(int32_t* prog, int nprog, uint32_t* maxdiv, int nmd, i
   for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nmd; t += stride) maxdiv[t] = 0u;
   for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nsim; t += stride) ctrl[4 * t + 1] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ticket[0] = 0;
-    ticket[1] = 0;
+    ticket[0] = 0;  // main run
+    ticket[1] = 0;  // re-run
+    ticket[2] = 0;  // re-run requested by some simulation
   }
 }
 
-// Early exit of solveSingleGrid (fluid.go:175-177), one thread per simulation: the first
+// Early exit of solveSingleGrid (fluid.go:175-177), one wave per simulation: the first
 // sweep whose max |div| is below 1e-5 ends the solve; if that is not the last sweep,
-// request a re-run of k*+1 sweeps.
-template <class M> __global__ void k_gs_decide(M m, int nsim, int N) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// request a re-run of k*+1 sweeps (and flag the launch, rerun[0], for the re-run kernel).
+// Lane l scans sweeps l, l+64, ...; the wave's minimum is the first such sweep.
+template <class M> __global__ void k_gs_decide(M m, int nsim, int N, int32_t* rerun) {
+  const int e = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
+  const int lane = threadIdx.x & (LANES - 1);
   if (e >= nsim) return;
   const SimDesc& D = sim_desc(m, e);
-  int kept = N;
+  int first = N;
   if (D.early_exit)
-    for (int k = 0; k < N; k++)
+    for (int k = lane; k < N; k += LANES)
       if (__uint_as_float(D.maxdiv[k]) < 1e-5f) {
-        kept = k + 1;
+        first = k;
         break;
       }
-  D.ctrl[1] = kept < N ? kept : 0;
+  for (int o = LANES / 2; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o, LANES));
+  if (lane == 0) {
+    const int kept = first < N ? first + 1 : N;
+    D.ctrl[1] = kept < N ? kept : 0;
+    if (kept < N) atomicOr(rerun, 1);
+  }
 }
 
 static bool tile_empty_h(int g, int kb, int K, int TK, int NX) {
@@ -586,7 +597,7 @@ static int layout(Ctx* c, int N) {
   FL_HIP(hipMalloc(&W.d_kout, sizeof(float) * W.kout_elems));
   FL_HIP(hipMalloc(&W.d_gout, sizeof(float) * W.gout_elems));
   FL_HIP(hipMalloc(&W.d_prog, sizeof(int32_t) * 2 * nt));
-  FL_HIP(hipMalloc(&W.d_ticket, sizeof(int32_t) * 2));
+  FL_HIP(hipMalloc(&W.d_ticket, sizeof(int32_t) * 4));  // main, re-run, re-run flag
   FL_HIP(hipMemcpy(W.d_tile, tiles.data(), sizeof(int2) * W.ntiles, hipMemcpyHostToDevice));
   W.grid = tk == 16 ? occupancy_grid<16>(c->dev, W.ntiles, false) : occupancy_grid<8>(c->dev, W.ntiles, false);
   return FL_OK;
@@ -617,16 +628,19 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.rPin = B_P;  // then p and T3 trade places
   R.rPout = B_T3;
   R.defer = defer_handoff(c->prm, W.ntiles, W.grid);
-  for (int which = 0; which < 2; which++) {
+  // without the early exit no re-run can be requested: the decision and re-run are skipped
+  const int runs = c->prm.early_exit ? 2 : 1;
+  for (int which = 0; which < runs; which++) {
     R.which = which;
     R.ticket = W.d_ticket + which;
+    R.rerun = which ? W.d_ticket + 2 : nullptr;
     if (W.tk == 16)
       hipLaunchKernelGGL((k_gs_wave<16, OneSim>), dim3(W.grid), dim3(16 * LANES), 0, c->stream, m, R);
     else
       hipLaunchKernelGGL((k_gs_wave<8, OneSim>), dim3(W.grid), dim3(8 * LANES), 0, c->stream, m, R);
     FL_HIP(hipGetLastError());
-    if (which == 0) {
-      hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(64), 0, c->stream, m, 1, N);
+    if (which == 0 && runs > 1) {
+      hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(LANES), 0, c->stream, m, 1, N, W.d_ticket + 2);
       FL_HIP(hipGetLastError());
     }
   }
@@ -701,7 +715,7 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   FL_HIP(hipMalloc(&B->kout, sizeof(float) * std::max<size_t>(kelems, 1)));
   FL_HIP(hipMalloc(&B->gout, sizeof(float) * std::max<size_t>(gelems, 1)));
   FL_HIP(hipMalloc(&B->prog, sizeof(int32_t) * std::max<size_t>(nprog, 1)));
-  FL_HIP(hipMalloc(&B->ticket, sizeof(int32_t) * 2));
+  FL_HIP(hipMalloc(&B->ticket, sizeof(int32_t) * 4));  // main, re-run, re-run flag
   FL_HIP(hipMemcpy(B->d_tiles, tiles.data(), sizeof(int2) * B->ntiles, hipMemcpyHostToDevice));
   size_t ko = 0, go = 0, po = 0;
   B->gs_fields.assign(descs.size(), SimDesc{});
@@ -743,16 +757,20 @@ int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
   R.rPin = rPin;
   R.rPout = rPout;
   R.defer = defer_handoff(B->prm, B->ntiles, B->grid);
-  for (int which = 0; which < 2; which++) {
+  const int runs = B->prm.early_exit ? 2 : 1;  // no early exit: nothing to decide or re-run
+  for (int which = 0; which < runs; which++) {
     R.which = which;
     R.ticket = B->ticket + which;
+    R.rerun = which ? B->ticket + 2 : nullptr;
     if (B->tk == 16)
       hipLaunchKernelGGL((k_gs_wave<16, ManySims>), dim3(B->grid), dim3(16 * LANES), 0, B->stream, m, R);
     else
       hipLaunchKernelGGL((k_gs_wave<8, ManySims>), dim3(B->grid), dim3(8 * LANES), 0, B->stream, m, R);
     FL_HIP(hipGetLastError());
-    if (which == 0) {
-      hipLaunchKernelGGL(k_gs_decide<ManySims>, dim3((E + 63) / 64), dim3(64), 0, B->stream, m, E, N);
+    if (which == 0 && runs > 1) {
+      // one wave per member, four per workgroup
+      hipLaunchKernelGGL(k_gs_decide<ManySims>, dim3((E + 3) / 4), dim3(4 * LANES), 0, B->stream, m, E, N,
+                         B->ticket + 2);
       FL_HIP(hipGetLastError());
     }
   }

@@ -799,12 +799,12 @@ static int vjp_step(Vjp* J, float dt, float* tp, int* kept) {
              (rc = dcopy(J, tp + T_M1 * N, c->M))))
     return rc;
   if (P.use_bfecc) {
-    if ((rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt))) return rc;
+    if ((rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt, false))) return rc;
     if (tp && ((rc = dcopy(J, tp + T_FU * N, c->U2)) || (rc = dcopy(J, tp + T_FV * N, c->V2)))) return rc;
     if ((rc = launch_bfecc_velocity_corr(c, c->U, c->V, c->U2, c->V2, c->T0, c->T1, dt))) return rc;
     if (tp && ((rc = dcopy(J, tp + T_CU * N, c->T0)) || (rc = dcopy(J, tp + T_CV * N, c->T1)))) return rc;
     if ((rc = launch_advect_velocity(c, c->T0, c->T1, c->U, c->V, dt))) return rc;
-    if ((rc = launch_advect_smoke(c, c->M, c->M2, dt))) return rc;
+    if ((rc = launch_advect_smoke(c, c->M, c->M2, dt, false))) return rc;
     if (tp && (rc = dcopy(J, tp + T_FM * N, c->M2))) return rc;
     if ((rc = launch_bfecc_smoke_corr(c, c->M, c->M2, c->T2, dt))) return rc;
     if (tp && (rc = dcopy(J, tp + T_CM * N, c->T2))) return rc;

@@ -215,10 +215,10 @@ static int enqueue_step(Batch* B, float dt) {
   // forward (or only) pass: from the current velocity into the other pair of roles
   const int fu = cu == B_U ? B_U2 : B_U, fv = cv == B_V ? B_V2 : B_V;
   if (P.use_bfecc) {
-    if ((rc = batch_advect_velocity(B, cu, cv, fu, fv, dt))) return rc;
+    if ((rc = batch_advect_velocity(B, cu, cv, fu, fv, dt, false))) return rc;
     if ((rc = batch_bfecc_velocity_corr(B, cu, cv, fu, fv, dt))) return rc;
     if ((rc = batch_advect_velocity(B, B_T0, B_T1, B_U, B_V, dt))) return rc;
-    if ((rc = batch_advect_smoke(B, B_M, B_M2, dt))) return rc;
+    if ((rc = batch_advect_smoke(B, B_M, B_M2, dt, false))) return rc;
     if ((rc = batch_bfecc_smoke_corr(B, dt))) return rc;
     return batch_advect_smoke(B, B_T2, B_M, dt);
   }

@@ -189,18 +189,20 @@ static int advect_smoke_sl(Ctx* c, float dt) {
   std::swap(c->M, c->M2);
   return FL_OK;
 }
-// advectVelocityBFECC (fluid.go:911-994): forward into U2/V2 (also newU/newV), backward +
-// correction + clamp into T0/T1, final forward pass from the corrected field into U/V.
+// advectVelocityBFECC (fluid.go:911-994): forward into U2/V2, backward + correction + clamp
+// into T0/T1, final forward pass from the corrected field into U/V and newU/newV. The forward
+// pass leaves newU/newV alone: the final pass writes every cell of them and reads them only
+// at faces neither pass advects, whose newU/newV the forward pass would keep as they are.
 static int advect_velocity_bfecc(Ctx* c, float dt) {
   int rc;
-  if ((rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt))) return rc;
+  if ((rc = launch_advect_velocity(c, c->U, c->V, c->U2, c->V2, dt, false))) return rc;
   if ((rc = launch_bfecc_velocity_corr(c, c->U, c->V, c->U2, c->V2, c->T0, c->T1, dt))) return rc;
   return launch_advect_velocity(c, c->T0, c->T1, c->U, c->V, dt);
 }
 // advectSmokeBFECC (fluid.go:997-1051)
 static int advect_smoke_bfecc(Ctx* c, float dt) {
   int rc;
-  if ((rc = launch_advect_smoke(c, c->M, c->M2, dt))) return rc;
+  if ((rc = launch_advect_smoke(c, c->M, c->M2, dt, false))) return rc;
   if ((rc = launch_bfecc_smoke_corr(c, c->M, c->M2, c->T2, dt))) return rc;
   return launch_advect_smoke(c, c->T2, c->M, dt);
 }

@@ -284,11 +284,14 @@ int launch_curl(Ctx* c, float* curl);
 int launch_conf_turb(Ctx* c, const float* curl, float dt, int do_conf, int do_turb);
 int launch_borders(Ctx* c);
 // SL velocity advection: trace with (Ut,Vt), sample (Us,Vs); write nU/nV and (Uo,Vo).
-int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt);
+// write_new = false: leave newU/newV (newM) alone. Only for BFECC's forward pass, whose final
+// pass rewrites them at every cell and reads them only where the forward pass did not change them.
+int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt,
+                           bool write_new = true);
 // BFECC backward pass fused with the correction and the clamp.
 int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd,
                                const float* Vfwd, float* Ucorr, float* Vcorr, float dt);
-int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt);
+int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt, bool write_new = true);
 int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt);
 int launch_apply_edits(Ctx* c, int nseg);
 int launch_apply_sources(Ctx* c);
@@ -304,9 +307,9 @@ int batch_flags(Batch* B);
 int batch_viscosity(Batch* B, float dt);
 int batch_conf_turb(Batch* B, float dt, int rU, int rV);
 int batch_borders(Batch* B, int rU, int rV);
-int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt);
+int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt, bool write_new = true);
 int batch_bfecc_velocity_corr(Batch* B, int rUo, int rVo, int rUf, int rVf, float dt);
-int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt);
+int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt, bool write_new = true);
 int batch_bfecc_smoke_corr(Batch* B, float dt);
 int batch_apply_sources(Batch* B);
 int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const int32_t* seg, const int32_t* segsim,

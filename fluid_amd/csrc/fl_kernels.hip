@@ -357,7 +357,7 @@ __device__ __forceinline__ float avg_v(const float* __restrict__ V, int c, int n
 // reads of cells outside a face's geometric range use a base cell inside the grid; samples of
 // faces that are not advected clamp like any other), so a wave waits for two rounds of loads
 // (the face neighbourhood, then both bilinear samples) instead of one round per branch.
-template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int rUo, int rVo, float dt) {
+template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int rUo, int rVo, float dt, int write_new) {
   const SimDesc* d;
   int i, j;
   if (!cell_of(m, d, i, j)) return;
@@ -394,16 +394,18 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
     rv = sample_field(xv, yv, Vt, g.hs, 0.0f, g, &badv);
   }
   const float ou = au ? ru : su, ov = av ? rv : sv;
-  nU[c] = ou;
+  if (write_new) {
+    nU[c] = ou;
+    nV[c] = ov;
+  }
   d->buf[rUo][c] = ou;
-  nV[c] = ov;
   d->buf[rVo][c] = ov;
   if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
 }
-int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt) {
+int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt, bool write_new) {
   const int a = role_of(c, Ut), b = role_of(c, Vt), e = role_of(c, Uo), f = role_of(c, Vo);
   launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_advect_velocity<OneSim>, g, bl, 0, s, m, a, b, e, f, dt);
+    hipLaunchKernelGGL(k_advect_velocity<OneSim>, g, bl, 0, s, m, a, b, e, f, dt, (int)write_new);
   });
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -495,7 +497,8 @@ int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, c
 
 // advectSmoke (fluid.go:400-434) using the current (already advected) velocity. Loads are
 // issued unconditionally from in-range addresses (ring cells read around cell (1,1)).
-template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, float visc) {
+template <class M>
+__global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, float visc, int write_new) {
   const SimDesc* d;
   int i, j;
   if (!cell_of(m, d, i, j)) return;
@@ -525,15 +528,15 @@ template <class M> __global__ void k_advect_smoke(M m, int rMs, int rMo, float d
     val += sd * nb;
   }
   const float out = adv ? go_maxf(val, 0.0f) : keep;
-  nM[c] = out;
+  if (write_new) nM[c] = out;
   d->buf[rMo][c] = out;
   if (adv && badl) atomicOr(d->ctrl + 2, 1);
 }
-int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt) {
+int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt, bool write_new) {
   const int a = role_of(c, Msrc), b = role_of(c, Mo);
   const float sa = c->prm.smoke_advection, visc = c->prm.viscosity_diffusion;
   launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_advect_smoke<OneSim>, g, bl, 0, s, m, a, b, dt, sa, visc);
+    hipLaunchKernelGGL(k_advect_smoke<OneSim>, g, bl, 0, s, m, a, b, dt, sa, visc, (int)write_new);
   });
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -636,8 +639,9 @@ int batch_borders(Batch* B, int rU, int rV) {
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
-int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt) {
-  hipLaunchKernelGGL(k_advect_velocity<ManySims>, FL_BATCH_CELLS(B), many(B), rUt, rVt, rUo, rVo, dt);
+int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt, bool write_new) {
+  hipLaunchKernelGGL(k_advect_velocity<ManySims>, FL_BATCH_CELLS(B), many(B), rUt, rVt, rUo, rVo, dt,
+                     (int)write_new);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -647,9 +651,9 @@ int batch_bfecc_velocity_corr(Batch* B, int rUo, int rVo, int rUf, int rVf, floa
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
-int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt) {
+int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt, bool write_new) {
   hipLaunchKernelGGL(k_advect_smoke<ManySims>, FL_BATCH_CELLS(B), many(B), rMs, rMo, dt, B->prm.smoke_advection,
-                     B->prm.viscosity_diffusion);
+                     B->prm.viscosity_diffusion, (int)write_new);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

@@ -151,7 +151,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     const int k0 = kb * TK;
     const int k = k0 + wave;
     const bool wave_on = k < K;
-    const bool last = k == K - 1;
+    // wave-uniform in a scalar register: the other waves skip the output block with one branch
+    const int last = __builtin_amdgcn_readfirstlane((int)(k == K - 1));
     const int i0 = LANES * g - k0;           // row of (lane 0, wave 0)
     const int row = i0 + lane - wave;        // this thread's row
     const bool real_row = row >= 0 && row <= NX - 1;
@@ -333,7 +334,6 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         const int s = blk * BLK + u;
         const int pb = (s + 1) & 1;  // buffer written at step s-1
         const int wb = s & 1;
-        const int j = s - lane - wave;
         // ---- inputs
         const float4 gin = Gst[cb][u][wave];
         float Ul = __int_as_float(
@@ -414,6 +414,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
           st_dev(kr + 2 * (LANES + 1) + lane + 1, p);
         }
         if (last) {  // only the wave of the final sweep writes the results
+          const int j = s - lane - wave;
           if (real_row && j >= 0 && j <= NY - 1) {
             const size_t c = (size_t)row * NY + j;
             const bool tu = row >= 1 && j >= 1 && j <= NY - 2;

@@ -211,6 +211,7 @@ def bench_diff(args):
     f = C.diff_scene(lambda *a: Fluid(*a), size, size, iters)
     f.Confinement = args.confinement
     f.BlockSweeps = args.block_sweeps
+    f.HandoffLag = args.handoff_lag
     shape = (f.NumX, f.NumY)
     T = C.gaussian_blobs(f.NumX, f.NumY)
     z = np.zeros(shape, dtype=np.float32)
@@ -276,6 +277,7 @@ def main():
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--solver", type=int, default=0)
     ap.add_argument("--block-sweeps", type=int, default=0)
+    ap.add_argument("--handoff-lag", type=int, default=0, help="0 auto, 1 on time, 2 one block late")
     ap.add_argument("--precise", action="store_true",
                     help="PreciseAdvection: back-trace in cell units (an addition; default is the reference's arithmetic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -310,6 +312,7 @@ def main():
     f = build(args.config, lambda *a: Fluid(*a), args.size, args.iters)
     f.Solver = args.solver
     f.BlockSweeps = args.block_sweeps
+    f.HandoffLag = args.handoff_lag
     f.PreciseAdvection = args.precise
     W, H = f.NumX - 2, f.NumY - 2
     cells = W * H

@@ -322,6 +322,8 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
   size_t ncells = 0, ndiag = 0;
   for (int e = 0; e < n; e++) {
     if (widths[e] < 1 || heights[e] < 1) return fail(FL_ERR_INVALID_ARG, "member sizes must be >= 1");
+    if ((int64_t)(widths[e] + 2) * (heights[e] + 2) > (int64_t)1 << 28)
+      return fail(FL_ERR_INVALID_ARG, "member grid too large (more than 2^28 cells)");
     ncells += (size_t)(widths[e] + 2) * (heights[e] + 2);
     ndiag += (size_t)(widths[e] + heights[e] + 3) * (widths[e] + 2);
   }

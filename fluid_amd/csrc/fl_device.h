@@ -98,6 +98,20 @@ __device__ __forceinline__ float turb_noise(int k) {
   return (float)go_sin((double)seed);
 }
 
+// A field element at a 32-bit cell index from a kernel-uniform base. The byte offset stays a
+// 32-bit unsigned value, so the compiler addresses with the base in scalar registers instead of
+// per-lane 64-bit arithmetic. Grids are limited to 2^28 cells (fl_create, fl_batch_create), so
+// byte offsets stay below 2^30.
+__device__ __forceinline__ float ld_cell(const float* __restrict__ base, int idx) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (uint32_t)idx * 4u);
+}
+__device__ __forceinline__ void st_cell(float* __restrict__ base, int idx, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + (uint32_t)idx * 4u) = v;
+}
+__device__ __forceinline__ uint32_t ld_flag(const uint8_t* __restrict__ base, int idx) {
+  return *(base + (uint32_t)idx);
+}
+
 // int(math.Floor(float64(v))) with NaN/overflow mapped to 0 and flagged (the
 // reference would panic on the resulting slice index).
 __device__ __forceinline__ int floor_index(float v, int* bad) {
@@ -131,7 +145,8 @@ __device__ __forceinline__ float sample_field(float x, float y, const float* __r
   float sx = 1.0f - tx;
   float sy = 1.0f - ty;
   const int n = g.NY;
-  float a = f[x0 * n + y0], b = f[x1 * n + y0], c = f[x1 * n + y1], d = f[x0 * n + y1];
+  float a = ld_cell(f, x0 * n + y0), b = ld_cell(f, x1 * n + y0), c = ld_cell(f, x1 * n + y1),
+        d = ld_cell(f, x0 * n + y1);
   return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
 }
 
@@ -194,7 +209,8 @@ __device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, 
   float sx = 1.0f - tx;
   float sy = 1.0f - ty;
   const int n = g.NY;
-  float a = f[x0 * n + y0], b = f[x1 * n + y0], c = f[x1 * n + y1], d = f[x0 * n + y1];
+  float a = ld_cell(f, x0 * n + y0), b = ld_cell(f, x1 * n + y0), c = ld_cell(f, x1 * n + y1),
+        d = ld_cell(f, x0 * n + y1);
   return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
 }
 

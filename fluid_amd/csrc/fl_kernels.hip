@@ -344,10 +344,10 @@ int launch_borders(Ctx* c) {
 
 // ---- semi-Lagrangian advection (fluid.go:291-434) ----------------------------------------
 __device__ __forceinline__ float avg_u(const float* __restrict__ U, int c, int n) {
-  return (U[c - 1] + U[c] + U[c + n - 1] + U[c + n]) * 0.25f;
+  return (ld_cell(U, c - 1) + ld_cell(U, c) + ld_cell(U, c + n - 1) + ld_cell(U, c + n)) * 0.25f;
 }
 __device__ __forceinline__ float avg_v(const float* __restrict__ V, int c, int n) {
-  return (V[c - n] + V[c] + V[c - n + 1] + V[c + 1]) * 0.25f;
+  return (ld_cell(V, c - n) + ld_cell(V, c) + ld_cell(V, c - n + 1) + ld_cell(V, c + 1)) * 0.25f;
 }
 
 // advectVelocity: copyBorder(newU,U), the advected faces, stale newU elsewhere; then U=newU.
@@ -368,14 +368,14 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
   float* nU = d->buf[B_NU];
   float* nV = d->buf[B_NV];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = d->flags[c];
+  const uint32_t f = ld_flag(d->flags, c);
   // geometric ranges of the advected faces (u: 1 <= j <= NY-2, v: 1 <= i <= NX-2, both i, j >= 1)
   const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
   const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
   const int cu = gu ? c : n + 1, cv = gv ? c : n + 1;  // (1,1): its neighbourhood is in range
-  const float uc = Ut[c], vc = Vt[c];
+  const float uc = ld_cell(Ut, c), vc = ld_cell(Vt, c);
   const float v_at_u = avg_v(Vt, cu, n), u_at_v = avg_u(Ut, cv, n);
-  const float su = ring ? uc : nU[c], sv = ring ? vc : nV[c];  // kept where not advected
+  const float su = ring ? uc : ld_cell(nU, c), sv = ring ? vc : ld_cell(nV, c);  // kept where not advected
   int badu = 0, badv = 0;
   float ru, rv;
   if (g.precise) {
@@ -395,11 +395,11 @@ template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int 
   }
   const float ou = au ? ru : su, ov = av ? rv : sv;
   if (write_new) {
-    nU[c] = ou;
-    nV[c] = ov;
+    st_cell(nU, c, ou);
+    st_cell(nV, c, ov);
   }
-  d->buf[rUo][c] = ou;
-  d->buf[rVo][c] = ov;
+  st_cell(d->buf[rUo], c, ou);
+  st_cell(d->buf[rVo], c, ov);
   if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
 }
 int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt, bool write_new) {
@@ -419,12 +419,12 @@ struct NbRange {
   float lo, hi;
 };
 __device__ __forceinline__ NbRange neighbor_range(const float* __restrict__ src, int c, int n) {
-  NbRange r{src[c], src[c]};
+  NbRange r{ld_cell(src, c), ld_cell(src, c)};
 #pragma unroll
   for (int di = -1; di <= 1; di++)
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++) {
-      float v = src[c + di * n + dj];
+      float v = ld_cell(src, c + di * n + dj);
       if (v < r.lo) r.lo = v;
       if (v > r.hi) r.hi = v;
     }
@@ -451,13 +451,13 @@ __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, i
   const float* Uf = d->buf[rUf];
   const float* Vf = d->buf[rVf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = d->flags[c];
+  const uint32_t f = ld_flag(d->flags, c);
   const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
   const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
   const int cu = gu ? c : n + 1, cv = gv ? c : n + 1, ci = ring ? n + 1 : c;
-  const float ou = Uo[c], ov = Vo[c];
+  const float ou = ld_cell(Uo, c), ov = ld_cell(Vo, c);
   const float v_at_u = avg_v(Vo, cu, n), u_at_v = avg_u(Uo, cv, n);
-  const float fu = Uf[c], fv = Vf[c];
+  const float fu = ld_cell(Uf, c), fv = ld_cell(Vf, c);
   const NbRange nu = neighbor_range(Uo, ci, n), nv = neighbor_range(Vo, ci, n);
   int badu = 0, badv = 0;
   float ru, rv;
@@ -480,8 +480,8 @@ __global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, i
   // interior cells: corr = orig - (bwd - orig)/2 clamped to the 3x3 of orig; the ring keeps orig
   const float kcu = clamp_to(ou - (bu - ou) * 0.5f, nu);
   const float kcv = clamp_to(ov - (bv - ov) * 0.5f, nv);
-  d->buf[rUc][c] = ring ? ou : kcu;
-  d->buf[rVc][c] = ring ? ov : kcv;
+  st_cell(d->buf[rUc], c, ring ? ou : kcu);
+  st_cell(d->buf[rVc], c, ring ? ov : kcv);
   if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
 }
 int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd, const float* Vfwd,
@@ -509,15 +509,16 @@ __global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, 
   const float* Ms = d->buf[rMs];
   float* nM = d->buf[B_NM];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = d->flags[c];
+  const uint32_t f = ld_flag(d->flags, c);
   const bool adv = (f & CF_INTERIOR) && (f & CF_FLUID);
   const int ci = ring ? n + 1 : c;
-  const float mc = Ms[c];
-  const float keep = ring ? mc : nM[c];
-  float u = (U[ci] + U[ci + n]) * 0.5f * smokeAdv;
-  float v = (V[ci] + V[ci + 1]) * 0.5f * smokeAdv;
+  const float mc = ld_cell(Ms, c);
+  const float keep = ring ? mc : ld_cell(nM, c);
+  float u = (ld_cell(U, ci) + ld_cell(U, ci + n)) * 0.5f * smokeAdv;
+  float v = (ld_cell(V, ci) + ld_cell(V, ci + 1)) * 0.5f * smokeAdv;
   float nb = 0.0f;
-  if (visc > 0.0f) nb = Ms[ci - n] + Ms[ci + n] + Ms[ci - 1] + Ms[ci + 1] - 4.0f * Ms[ci];
+  if (visc > 0.0f)
+    nb = ld_cell(Ms, ci - n) + ld_cell(Ms, ci + n) + ld_cell(Ms, ci - 1) + ld_cell(Ms, ci + 1) - 4.0f * ld_cell(Ms, ci);
   int badl = 0;
   float x = (float)i * g.h + g.hs - dt * u;
   float y = (float)j * g.h + g.hs - dt * v;
@@ -528,8 +529,8 @@ __global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, 
     val += sd * nb;
   }
   const float out = adv ? go_maxf(val, 0.0f) : keep;
-  if (write_new) nM[c] = out;
-  d->buf[rMo][c] = out;
+  if (write_new) st_cell(nM, c, out);
+  st_cell(d->buf[rMo], c, out);
   if (adv && badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt, bool write_new) {
@@ -555,12 +556,12 @@ template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int
   const float* Mo = d->buf[rMo];
   const float* Mf = d->buf[rMf];
   const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = d->flags[c];
+  const uint32_t f = ld_flag(d->flags, c);
   const bool adv = !ring && (f & CF_FLUID);
   const int ci = ring ? n + 1 : c;
-  const float om = Mo[c];
-  float u = (U[ci] + U[ci + n]) * 0.5f * smokeAdv;
-  float v = (V[ci] + V[ci + 1]) * 0.5f * smokeAdv;
+  const float om = ld_cell(Mo, c);
+  float u = (ld_cell(U, ci) + ld_cell(U, ci + n)) * 0.5f * smokeAdv;
+  float v = (ld_cell(V, ci) + ld_cell(V, ci + 1)) * 0.5f * smokeAdv;
   const NbRange nr = neighbor_range(Mo, ci, n);
   int badl = 0;
   float x = (float)i * g.h + g.hs + dt * u;
@@ -570,7 +571,7 @@ template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int
   const float bm = adv ? bs : 0.0f;  // solid interior cells: the backward buffer's zero
   float cm = clamp_to(om - (bm - om) * 0.5f, nr);
   if (cm < 0.0f) cm = 0.0f;
-  d->buf[rMc][c] = ring ? om : cm;
+  st_cell(d->buf[rMc], c, ring ? om : cm);
   if (adv && badl) atomicOr(d->ctrl + 2, 1);
 }
 int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt) {

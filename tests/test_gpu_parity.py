@@ -90,6 +90,21 @@ def test_pass_bit_exact(w, h, name, dt, iters):
     assert_same(g, o, name)
 
 
+@pytest.mark.parametrize("w,h", SIZES)
+@pytest.mark.parametrize("iters", [1, 13, 40])
+def test_gs_late_handoff_bit_exact(w, h, iters):
+    """The temporal-blocked solve with its handoff rows published a block late (HandoffLag 2),
+    which the default picks only on large grids, on small and single-tile grids."""
+    g, o = pair(3.0, w, h, 0.1)
+    load_both(g, o, random_state(w + 2, h + 2, seed=w * 17 + h + iters))
+    for f in (g, o):
+        knobs(f, PressureDamping=0.8, Relaxation=1.7, EarlyExit=False)
+    g.Solver, g.HandoffLag = 2, 2
+    for f in (g, o):
+        f.run_pass("makeIncompressible", 0.1, iters)
+    assert_same(g, o, "late handoff %dx%d, %d sweeps" % (w, h, iters))
+
+
 @pytest.mark.parametrize("solver", [1, 2], ids=["hyper", "wave"])
 @pytest.mark.parametrize("opts", [
     dict(), dict(UseBFECC=True), dict(Confinement=0.3, UseBFECC=True),
@@ -110,7 +125,8 @@ def test_simulate_bit_exact(solver, opts):
 
 
 @pytest.mark.parametrize("solver", [1, 2], ids=["hyper", "wave"])
-def test_early_exit_rollback(solver):
+@pytest.mark.parametrize("lag", [0, 2], ids=["auto_lag", "late_handoff"])
+def test_early_exit_rollback(solver, lag):
     """A quiescent field converges in sweep 0: the device must keep exactly one sweep."""
     g, o = pair(1000.0, 40, 30, 0.01)
     st = {k: np.zeros((42, 32), np.float32) for k in FIELDS}
@@ -119,7 +135,7 @@ def test_early_exit_rollback(solver):
     load_both(g, o, st)
     for f in (g, o):
         knobs(f, PressureIters=25)
-    g.Solver = solver
+    g.Solver, g.HandoffLag = solver, lag
     # small divergence that the first sweeps remove below 1e-5 after a few sweeps
     U = np.zeros((42, 32), np.float32)
     U[20, 15] = 3e-5

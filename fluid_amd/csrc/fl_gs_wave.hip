@@ -19,8 +19,9 @@
 // Handoff data and progress counters are accessed only with agent-scope relaxed atomic
 // loads and stores (device-coherent accesses, global_load/store sc1), so neither side
 // needs an L2 writeback or invalidate; the producer waits for its data stores to
-// complete (s_waitcnt vmcnt(0)) before it stores the new progress value, and consumers
-// issue their data loads only after observing it.
+// complete before it stores the new progress value (publish_rows: a full wait at the next
+// block, or a counted wait one block later), and consumers issue their data loads only after
+// observing it.
 //
 // Cells outside the interior or solid pass their faces through unchanged, exactly like
 // the skipped cells of the serial loop. Faces no interior cell touches keep their input
@@ -76,9 +77,9 @@ __device__ __forceinline__ float4 ld_dev3(const float* p, int stride) {
 }
 // Wait until every store this wave issued has been acknowledged by the memory system.
 __device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-// Wait until at most N of this wave's vector memory operations are in flight. The counter
-// retires loads and stores in issue order, so every operation issued before the N most
-// recent ones is then complete.
+// Wait until at most N of this wave's vector memory operations are in flight. The wave's
+// stores retire in issue order, so when at least N stores were issued after a given store,
+// that store is complete once this returns (loads issued in between only lengthen the wait).
 template <int N> __device__ __forceinline__ void wait_stores_but() {
   static_assert(N >= 0 && N <= 63, "vmcnt is a 6-bit count");
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
@@ -247,7 +248,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         const int it = tid + q * NT;
         if (it < KITEM) {
           float4 v = stage[q];
-          const bool lane0 = kb > 0 && it % (LANES + 1) == 0;  // selects read both sides on every path
+          // lane 0 of each K-stream row takes V and p of the corner producer (zero without one)
+          const bool lane0 = kb > 0 && it % (LANES + 1) == 0;
           v.y = lane0 ? corner[q].x : v.y;
           v.z = lane0 ? corner[q].y : v.z;
           Kst[buf][it / (LANES + 1)][it % (LANES + 1)] = v;

@@ -31,21 +31,11 @@ enum : uint32_t {
   CF_INTERIOR = 1u << 6 // 1 <= i <= NX-2 and 1 <= j <= NY-2
 };
 
-// Go builtin min/max on float32: NaN propagates, min(-0,+0) = -0, max(-0,+0) = +0.
-__device__ __forceinline__ float go_minf(float a, float b) {
-  if (a != a) return a;
-  if (b != b) return b;
-  if (a < b) return a;
-  if (b < a) return b;
-  return __builtin_signbit(a) ? a : b;
-}
-__device__ __forceinline__ float go_maxf(float a, float b) {
-  if (a != a) return a;
-  if (b != b) return b;
-  if (a > b) return a;
-  if (b > a) return b;
-  return __builtin_signbit(a) ? b : a;
-}
+// Go builtin min/max on float32: NaN propagates, min(-0,+0) = -0, max(-0,+0) = +0. That is
+// IEEE 754-2019 minimum/maximum, one v_minimum3_f32 / v_maximum3_f32 on gfx950. (Which NaN
+// comes back is unspecified in both; NaN payloads already differ between x86 and the GPU.)
+__device__ __forceinline__ float go_minf(float a, float b) { return __builtin_elementwise_minimum(a, b); }
+__device__ __forceinline__ float go_maxf(float a, float b) { return __builtin_elementwise_maximum(a, b); }
 
 // Go math.Sin for 0 <= |x| < 2^29 (Cephes sin.c octant reduction and polynomials),
 // evaluated in double without contraction.

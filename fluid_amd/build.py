@@ -1,11 +1,14 @@
 """Build libfluidhip.so (gfx950) in-tree with hipcc.
 
 The HIP sources are compiled with contraction off (the reference's float32 arithmetic
-rounds every operation) and with packed FP32 instructions disabled.
+rounds every operation) and with packed FP32 instructions disabled, one object per source
+(in parallel, with per-source flags), then linked.
 """
 import os
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -18,7 +21,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-fno-gpu-flush-denormals-to-zero",
@@ -27,10 +29,15 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+# The temporal-blocked Gauss-Seidel kernel issues its row loads and stores in longer
+# clauses under the memory-clause scheduler: 2% fewer cycles per sweep at 4096^2 (measured
+# against the default, max-ilp and iterative-ilp strategies); the other sources gain nothing.
+PER_SOURCE = {
+    "fl_gs_wave.hip": ["-mllvm", "--amdgpu-sched-strategy=max-memory-clause"],
+}
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -39,6 +46,18 @@ def build(verbose=False):
         raise RuntimeError("hipcc failed:\n" + out)
     if verbose and out.strip():
         print(out)
+
+
+def build(verbose=False):
+    tmp = LIB + ".tmp"
+    with tempfile.TemporaryDirectory() as d:
+        def compile_one(src):
+            obj = os.path.join(d, src + ".o")
+            _run([HIPCC] + FLAGS + PER_SOURCE.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+            return obj
+        with ThreadPoolExecutor(min(len(SOURCES), os.cpu_count() or 1)) as ex:
+            objs = list(ex.map(compile_one, SOURCES))
+        _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--handoff-lag", type=int, default=0, help="0 auto, 1 on time, 2 one block late")
     ap.add_argument("--precise", action="store_true",
                     help="PreciseAdvection: back-trace in cell units (an addition; default is the reference's arithmetic)")
+    ap.add_argument("--multigrid", action="store_true",
+                    help="UseMultigrid with 2 levels; --iters is then the number of V-cycles per step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     ap.add_argument("--calibrate", action="store_true",
@@ -314,6 +316,8 @@ def main():
     f.BlockSweeps = args.block_sweeps
     f.HandoffLag = args.handoff_lag
     f.PreciseAdvection = args.precise
+    if args.multigrid:
+        f.UseMultigrid, f.MultigridLevels = True, 2
     W, H = f.NumX - 2, f.NumY - 2
     cells = W * H
     for _ in range(args.warmup):
@@ -357,13 +361,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "%s %dx%d FP32, %d GS sweeps/step, BFECC %s, dt 0.1" % (
-            args.config, W, H, sweeps, "on" if f.UseBFECC else "off"), "grid": [W, H], "pressure_sweeps": sweeps,
-            "solver": ["auto", "hyperplane", "temporal-blocked"][args.solver],
+        "config": {"workload": "%s %dx%d FP32, %s/step, BFECC %s, dt 0.1" % (
+            args.config, W, H, ("%d V-cycles (%d fine GS sweeps, 40 coarse sweeps each)" % (args.iters, sweeps))
+            if args.multigrid else "%d GS sweeps" % sweeps, "on" if f.UseBFECC else "off"),
+            "grid": [W, H], "pressure_sweeps": sweeps,
+            "solver": "multigrid" if args.multigrid else ["auto", "hyperplane", "temporal-blocked"][args.solver],
             "advection": "precise (cell-unit back-trace)" if args.precise else "reference arithmetic"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.config, args.size, args.iters),
-                     "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},
+        "roofline": None if args.multigrid else {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(args.config, args.size, args.iters),
+            "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},
         "step_alg_bytes_per_cell": b_alg,
         "step_achieved_GBs": step_gbs,
         "step_frac_hbm": step_gbs / HBM_PEAK_GBS,

@@ -215,37 +215,125 @@ __global__ void k_mg_restrict(Grid g, int cNX, int cNY, const float* __restrict_
 }
 
 // solveCoarseGrid's 40 in-place lexicographic sweeps at relaxation 1.6 (fluid.go:686-716),
-// bit-identical by the hyperplane schedule t = i + j + 2k: a cell's four neighbours lie on
-// t-1 (already updated) and t+1 (not yet), and no two cells of one hyperplane touch the same
-// word. One workgroup walks the hyperplanes with a barrier between them.
-static constexpr int MG_COARSE_THREADS = 1024, MG_COARSE_ITERS = 40;
-__global__ __launch_bounds__(MG_COARSE_THREADS) void k_mg_coarse_gs(int cNX, int cNY, const float* __restrict__ cS,
-                                                                   const float* __restrict__ rhs, float* cP) {
-  const int W = cNX - 2, H = cNY - 2, K = MG_COARSE_ITERS;
-  if (W < 1 || H < 1) return;
-  const float relax = 1.6f;
-  const int tmax = W + H + 2 * (K - 1);  // t' = (i-1) + (j-1) + 2k in [0, tmax]
-  for (int t = 0; t <= tmax; t++) {
-    const int klo = max(0, (t - (W - 1) - (H - 1) + 1) / 2), khi = min(K - 1, t / 2);
-    const int span = W;  // i - 1 in [0, W)
-    for (int q = threadIdx.x; q < (khi - klo + 1) * span; q += blockDim.x) {
-      const int k = klo + q / span, ii = q % span;
-      const int jj = t - ii - 2 * k;
-      if (jj < 0 || jj >= H) continue;
-      const int i = ii + 1, j = jj + 1, c = i * cNY + j;
-      if (cS[c] == 0.0f) continue;
-      const float sx0 = cS[c - cNY], sx1 = cS[c + cNY], sy0 = cS[c - 1], sy1 = cS[c + 1];
-      const float s = sx0 + sx1 + sy0 + sy1;
-      if (s == 0.0f) continue;
-      const float pc = cP[c];
-      const float lap = sx0 * (cP[c - cNY] - pc) + sx1 * (cP[c + cNY] - pc) + sy0 * (cP[c - 1] - pc) +
-                        sy1 * (cP[c + 1] - pc);
-      const float res = rhs[c] - lap;
-      const float corr = -res / s * relax;
-      cP[c] = pc + corr;
-    }
-    __syncthreads();
+// bit-identical to the serial loop. The interior is cut into tiles (I, J) of 32 rows (i) by
+// 64 columns (j), 26 KB of LDS, six workgroups to a CU. Tile (I, J) runs sweep k in the launch
+// for diagonal T = I + J + 2k. Its four neighbour tiles then hold the values the serial order
+// reads: (I-1, J) and (I, J-1) have done sweep k (diagonal T-1), (I+1, J) and (I, J+1) sweep
+// k-1 but not k (T-1 and T+1). No two tiles of one diagonal are neighbours (their T would
+// differ by an odd number), so a tile reads its one-cell halo once at the start and writes its
+// interior back at the end. Inside, one wave walks the tile's anti-diagonals (the other three
+// waves help load and store the tile): lane l updates column l, row d - l at step d, after
+// (row - 1, l) and (row, l - 1) (step d - 1, this lane and lane l - 1) and before (row + 1, l)
+// and (row, l + 1). Every step runs the reference's expression on the same operands, so the
+// result is the serial loop's to the bit.
+static constexpr int MG_COARSE_ITERS = 40, MG_TR = 32, MG_TC = 64, MG_PITCH = MG_TC + 2;
+
+__global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int nI, int nJ, int T, int klo,
+                                                         const float* __restrict__ cS,
+                                                         const float* __restrict__ rhs, float* __restrict__ cP) {
+  const int k = klo + blockIdx.y, I = blockIdx.x, J = T - 2 * k - I;
+  if (J < 0 || J >= nJ) return;
+  __shared__ float lp[(MG_TR + 2) * MG_PITCH], ls[(MG_TR + 2) * MG_PITCH], lr[MG_TR * MG_TC];
+  const int W = cNX - 2, H = cNY - 2;
+  const int i0 = 1 + I * MG_TR, j0 = 1 + J * MG_TC;  // first interior cell of the tile
+  const int rows = min(MG_TR, W - I * MG_TR), cols = min(MG_TC, H - J * MG_TC);
+  const int tid = threadIdx.x, l = tid & 63;
+  // Load: rows i0-1 .. i0+rows and columns j0-1 .. j0+cols, all inside the grid (i0+rows <=
+  // W+1 = cNX-1), as one flat range over the four waves with every load in flight at once.
+  // The corners belong to diagonal tiles and are never read.
+  const int pw = cols + 2, n = (rows + 2) * pw;
+  constexpr int LU = ((MG_TR + 2) * MG_PITCH + 255) / 256;
+  float vp[LU], vs[LU], vr[LU];
+#pragma unroll
+  for (int u = 0; u < LU; u++) {
+    const int e = tid + 256 * u, r = e / pw - 1, q = e - (r + 1) * pw;
+    const bool in = e < n && !((r < 0 || r == rows) && (q == 0 || q == cols + 1));
+    const size_t g = (size_t)(i0 + r) * cNY + j0 - 1 + q;
+    vp[u] = in ? cP[g] : 0.0f;
+    vs[u] = in ? cS[g] : 0.0f;
+    vr[u] = (in && r >= 0 && r < rows && q >= 1 && q <= cols) ? rhs[g] : 0.0f;
   }
+#pragma unroll
+  for (int u = 0; u < LU; u++) {
+    const int e = tid + 256 * u, r = e / pw - 1, q = e - (r + 1) * pw;
+    if (e < n) {
+      lp[(r + 1) * MG_PITCH + q] = vp[u];
+      ls[(r + 1) * MG_PITCH + q] = vs[u];
+      if (r >= 0 && r < rows && q >= 1 && q <= cols) lr[r * MG_TC + q - 1] = vr[u];
+    }
+  }
+  __syncthreads();
+  // The walk. Lane l is at row r = d - l at step d (also before and after its active rows, so
+  // its registers are there for its neighbours) and holds its column's S[r-1..r+1], the old
+  // P[r], P[r+1], rhs[r] and cur = P[r-1] after this sweep. Lane l-1 (row r+1) passes on the
+  // new P[r][l-1] and S[r][l-1]; lane l+1 (row r-1) the old P[r][l+1] and S[r][l+1], by wave
+  // shifts in full-wave control flow. Lane 0 and lane 63 take the halo columns instead; for a
+  // narrower tile, lane `cols` walks the east halo column. LDS is only read here (the cells a
+  // lane will still read are never the ones written), so the walk needs no barrier; results
+  // go to LDS for the write-back.
+  if (tid < 64) {
+  const float relax = 1.6f;
+  auto row_of = [&](int x) { return min(max(x, -1), rows) + 1; };  // LDS row of row x, clamped
+  const int col = l + 1;                                                // LDS column of this lane
+  int r = -l;
+  float sm1 = ls[row_of(r - 1) * MG_PITCH + col], s0 = ls[row_of(r) * MG_PITCH + col];
+  float sp1 = ls[row_of(r + 1) * MG_PITCH + col];
+  float po = lp[row_of(r) * MG_PITCH + col], po1 = lp[row_of(r + 1) * MG_PITCH + col];
+  float cur = lp[row_of(r - 1) * MG_PITCH + col];
+  float rh = lr[min(max(r, 0), rows - 1) * MG_TC + l];
+  for (int d = 0; d < rows + cols - 1; d++, r++) {
+    const int hrow = row_of(r) * MG_PITCH;
+    const float pWh = lp[hrow], sWh = ls[hrow], pEh = lp[hrow + cols + 1], sEh = ls[hrow + cols + 1];
+    const float pW = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(pWh), __float_as_int(cur), 0x138, 0xF, 0xF, false));
+    const float sy0 = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(sWh), __float_as_int(sm1), 0x138, 0xF, 0xF, false));
+    const float pE = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(pEh), __float_as_int(po1), 0x130, 0xF, 0xF, false));
+    const float sy1 = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(sEh), __float_as_int(sp1), 0x130, 0xF, 0xF, false));
+    // next row's values (prefetched before the update's dependent chain)
+    const float sp2 = ls[row_of(r + 2) * MG_PITCH + col], po2 = lp[row_of(r + 2) * MG_PITCH + col];
+    const float rh1 = lr[min(max(r + 1, 0), rows - 1) * MG_TC + l];
+    float pn = po;
+    const bool active = l < cols && r >= 0 && r < rows;
+    if (active && s0 != 0.0f) {
+      const float sx0 = sm1, sx1 = sp1;
+      const float s = sx0 + sx1 + sy0 + sy1;
+      if (s != 0.0f) {
+        const float pc = po;
+        const float lap = sx0 * (cur - pc) + sx1 * (po1 - pc) + sy0 * (pW - pc) + sy1 * (pE - pc);
+        const float res = rh - lap;
+        const float corr = -res / s * relax;
+        pn = pc + corr;
+      }
+    }
+    if (active) lp[(r + 1) * MG_PITCH + col] = pn;
+    cur = pn;
+    sm1 = s0;
+    s0 = sp1;
+    sp1 = sp2;
+    po = po1;
+    po1 = po2;
+    rh = rh1;
+  }
+  }
+  __syncthreads();
+  for (int e = tid; e < rows * cols; e += 256) {
+    const int r = e / cols, q = e - r * cols;
+    cP[(size_t)(i0 + r) * cNY + j0 + q] = lp[(r + 1) * MG_PITCH + q + 1];
+  }
+}
+
+// All 40 coarse sweeps: one launch per tile diagonal, in order on the stream.
+static int mg_coarse_solve(Ctx* c, int cNX, int cNY) {
+  const int W = cNX - 2, H = cNY - 2, K = MG_COARSE_ITERS;
+  if (W < 1 || H < 1) return FL_OK;
+  const int nI = (W + MG_TR - 1) / MG_TR, nJ = (H + MG_TC - 1) / MG_TC;
+  for (int T = 0; T <= (nI - 1) + (nJ - 1) + 2 * (K - 1); T++) {
+    const int klo = std::max(0, (T - (nI - 1) - (nJ - 1) + 1) / 2), khi = std::min(K - 1, T / 2);
+    if (klo > khi) continue;  // no tile on this diagonal (a one-tile grid has none on odd T)
+    hipLaunchKernelGGL(k_mg_coarse_tiles, dim3(nI, khi - klo + 1), dim3(256), 0, c->stream, cNX, cNY, nI, nJ, T,
+                       klo, c->mg_S, c->mg_R, c->mg_P);
+  }
+  FL_HIP(hipGetLastError());
+  return FL_OK;
 }
 
 // prolongateCorrection (fluid.go:719-756) at one fine cell.
@@ -330,8 +418,7 @@ int multigrid_solve(Ctx* c, int numIters, float dt) {
     hipLaunchKernelGGL(k_mg_residual, fg, fb, 0, c->stream, c->grid, c->S, c->U, c->V, c->p, c->T0);
     hipLaunchKernelGGL(k_mg_restrict, cg, fb, 0, c->stream, c->grid, cNX, cNY, c->S, c->T0, c->mg_R, c->mg_S,
                        c->mg_P);
-    hipLaunchKernelGGL(k_mg_coarse_gs, dim3(1), dim3(MG_COARSE_THREADS), 0, c->stream, cNX, cNY, c->mg_S, c->mg_R,
-                       c->mg_P);
+    if ((rc = mg_coarse_solve(c, cNX, cNY))) return restore(), rc;
     hipLaunchKernelGGL(k_mg_apply, fg, fb, 0, c->stream, c->grid, cNX, cNY, c->S, c->mg_P, cp, c->U, c->V, c->p);
     FL_HIP(hipGetLastError());
     c->d_omega = c->mg_omega + 3;

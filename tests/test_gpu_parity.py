@@ -291,6 +291,23 @@ def test_multigrid_bit_exact(w, h, bfecc, scale):
     assert g.stats()["sweeps_run"] == o.last_sweeps()
 
 
+@pytest.mark.parametrize("w,h", [(260, 131), (300, 190)])
+@pytest.mark.parametrize("scene", ["cavity", "jet"])
+def test_multigrid_scenes_bit_exact(w, h, scene):
+    """The V-cycle on coarse grids of several tiles in both directions (260x131 leaves a 1-row
+    and a 1-column edge tile), from the presets: the random states diverge at these sizes."""
+    mk = C.cavity_scene if scene == "cavity" else C.jet_scene
+    g, o = mk(lambda *a: Fluid(*a), w, h, 4), mk(lambda *a: OracleFluid(*a), w, h, 4)
+    for f in (g, o):
+        f.UseMultigrid, f.MultigridLevels = True, 2
+    for step in range(6):
+        g.update(C.DT)
+        o.update(C.DT)
+        assert_same(g, o, "%s step %d" % (scene, step))
+    assert np.isfinite(g.get("U")).all() and np.isfinite(g.get("p")).all()
+    assert g.stats()["sweeps_run"] == o.last_sweeps()
+
+
 def test_multigrid_pass_and_converged_break():
     # the stand-alone makeIncompressible pass with the V-cycle
     g, o = pair(3.0, 50, 41, 0.1)

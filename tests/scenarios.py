@@ -235,6 +235,120 @@ def multigrid_performance(make, use_multigrid, iters):
     return f, {"velocities stay <= 200": ok}
 
 
+def diffusion_behavior(make):
+    """TestDiffusionBehavior (fluid_test.go:37-82): one makeIncompressible(10, 0.01) pass on a
+    spike spreads it to the four neighbours."""
+    f = make(1.0, 10, 10, 1.0)
+    _all_fluid(f)
+    c = (f.NumX // 2, f.NumY // 2)
+    _write(f, "U", [c], 10.0)
+    _write(f, "V", [c], 10.0)
+    f.run_pass("makeIncompressible", 0.01, 10)
+    U, V = f.get("U"), f.get("V")
+    checks = {"centre velocity reduced": float(U[c]) < 10.0}
+    for (i, j) in ((c[0] - 1, c[1]), (c[0] + 1, c[1]), (c[0], c[1] - 1), (c[0], c[1] + 1)):
+        if 0 < i < f.NumX - 1 and 0 < j < f.NumY - 1:
+            checks["neighbour %d,%d gained velocity" % (i, j)] = abs(float(U[i, j])) >= 0.05 or abs(float(V[i, j])) >= 0.05
+    return f, checks
+
+
+def _stable(f, limit):
+    mu, mv = _max_uv(f)
+    return np.isfinite(mu) and np.isfinite(mv) and mu <= limit and mv <= limit
+
+
+def bfecc_stability(make):
+    """TestBFECCStability (fluid_test.go:650-702): uniform (2, 1) flow, dt 0.2, 100 steps, a kick
+    at (5, 5) every 10 steps; the test leaves UseBFECC at its default."""
+    f = make(1.0, 20, 20, 1.0)
+    _all_fluid(f)
+    inner = [(i, j) for i in range(1, f.NumX - 1) for j in range(1, f.NumY - 1)]
+    _write(f, "U", inner, 2.0)
+    _write(f, "V", inner, 1.0)
+    ok = True
+    for step in range(100):
+        if not _stable(f, 1000.0):
+            ok = False
+            break
+        f.Simulate(0.2)
+        if step % 10 == 0:
+            _write(f, "U", [(5, 5)], 3.0)
+            _write(f, "V", [(5, 5)], 2.0)
+    return f, {"velocities stay <= 1000 and finite": ok}
+
+
+def bfecc_jet_stability(make):
+    """TestBFECCJetStability (fluid_test.go:704-758): a 6-row jet at 20 held at i = 1 for 30
+    steps of dt 0.1 (UseBFECC at its default)."""
+    f = make(1.0, 30, 20, 1.0)
+    _all_fluid(f)
+    rows, jx = jet_rows(f, 6), 1
+    ok = True
+    for _ in range(30):
+        _write(f, "U", [(jx, j) for j in rows], 20.0)
+        _write(f, "V", [(jx, j) for j in rows], 0.0)
+        if not _stable(f, 500.0):
+            ok = False
+            break
+        f.Simulate(0.1)
+    return f, {"velocities stay <= 500 and finite": ok}
+
+
+def performance_comparison(make, viscosity, damping, confinement):
+    """TestPerformanceComparison (fluid_test.go:841-899): a one-cell jet at 8 for 10 steps;
+    max(|U| + |V|) over every index stays <= 100."""
+    f = make(1.0, 15, 10, 1.0)
+    _all_fluid(f)
+    f.ViscosityDiffusion, f.PressureDamping, f.Confinement = viscosity, damping, confinement
+    for _ in range(10):
+        _write(f, "U", [(2, f.NumY // 2)], 8.0)
+        f.Simulate(0.1)
+    vel = np.abs(f.get("U")) + np.abs(f.get("V"))
+    mx = float(vel.max())
+    return f, {"max |U|+|V| <= 100 and finite": bool(np.isfinite(vel).all()) and mx <= 100.0}
+
+
+def advanced_visual_enhancements(make):
+    """TestAdvancedVisualEnhancements (fluid_test.go:901-1022): viscosity, damping, confinement,
+    turbulence and SmokeAdvection 1.2 together, three solid cells, 20 steps at the adaptive
+    time step from base 0.08."""
+    f = make(1.0, 25, 15, 1.0)
+    _all_fluid(f)
+    f.ViscosityDiffusion, f.PressureDamping, f.Confinement = 0.15, 0.92, 0.08
+    f.TurbulenceStrength, f.SmokeAdvection = 0.03, 1.2
+    c = f.NumY // 2
+    rows, jx = jet_rows(f, 6), 3
+    _write(f, "S", [(12, c), (18, c - 3), (18, c + 3)], 0.0)
+    ok = True
+    for _ in range(20):
+        dt = f.GetAdaptiveTimeStep(0.08)
+        _write(f, "U", [(jx, j) for j in rows], 12.0)
+        _write(f, "M", [(jx, j) for j in rows], 1.0)
+        f.Simulate(dt)
+        M = f.get("M")[1:-1, 1:-1]
+        if not (_stable(f, 100.0) and np.isfinite(M).all()):
+            ok = False
+    downstream = float(f.get("M")[jx + 5:-1, 1:-1].astype(np.float64).sum())
+    return f, {"velocities stay <= 100, no NaN": ok, "downstream smoke >= 2": downstream >= 2.0}
+
+
+def bfecc_accuracy(make, use_bfecc):
+    """TestBFECCAccuracy (fluid_test.go:1210-1268): a 5x5 smoke blob in a uniform flow of 3,
+    50 steps of dt 0.05 with the flow restored after each. The reference only logs the peak
+    smoke of the two methods; the check here is that it stays finite and within [0, 1]."""
+    f = make(1.0, 30, 30, 1.0)
+    _all_fluid(f)
+    inner = [(i, j) for i in range(1, f.NumX - 1) for j in range(1, f.NumY - 1)]
+    _write(f, "U", inner, 3.0)
+    _write(f, "M", [(i, j) for i in range(8, 13) for j in range(13, 18)], 1.0)
+    f.UseBFECC = use_bfecc
+    for _ in range(50):
+        f.Simulate(0.05)
+        _write(f, "U", inner, 3.0)
+    peak = float(f.get("M").max())
+    return f, {"peak smoke finite, in [0, 1]": np.isfinite(peak) and 0.0 <= peak <= 1.0}
+
+
 SCENARIOS = {
     "realistic_jet": realistic_jet,
     "obstacle_vortex_shedding": obstacle_vortex_shedding,
@@ -247,4 +361,12 @@ SCENARIOS = {
     "multigrid_performance_sg8": lambda mk: multigrid_performance(mk, False, 8),
     "multigrid_performance_mg4": lambda mk: multigrid_performance(mk, True, 4),
     "multigrid_performance_sg4": lambda mk: multigrid_performance(mk, False, 4),
+    "diffusion_behavior": diffusion_behavior,
+    "bfecc_stability": bfecc_stability,
+    "bfecc_jet_stability": bfecc_jet_stability,
+    "performance_comparison_enhanced": lambda mk: performance_comparison(mk, 0.1, 0.95, 0.05),
+    "performance_comparison_standard": lambda mk: performance_comparison(mk, 0.0, 1.0, 0.0),
+    "advanced_visual_enhancements": advanced_visual_enhancements,
+    "bfecc_accuracy_sl": lambda mk: bfecc_accuracy(mk, False),
+    "bfecc_accuracy_bfecc": lambda mk: bfecc_accuracy(mk, True),
 }

@@ -92,6 +92,55 @@ def ensemble_step(target, specs, step, members):
         t.update(C.DT)
 
 
+GO_PUBLISHED_MS = {"quiescent": 1.0, "jet": 4.6}  # Apple M4 Max, fluid_bench_test.go:19,29
+
+
+def bench_go(args):
+    """The reference's own Go benchmarks (fluid_bench_test.go:19-60) on the device: 300x251,
+    density 1000, h 0.01, dt 1/120, Simulate's 8 sweeps; "jet" and "bfecc" write U = 4 and
+    M = 1 on 100 cells each frame. Frames run back to back as in `go test -bench`; the rate is
+    interior cell-steps per second, and vs_baseline divides it by the rate of the frame times
+    the reference publishes for an Apple M4 Max (1 ms quiescent, 4.6 ms with the jet)."""
+    from fluid_amd import Fluid
+    from fluid_amd import configs as C
+    v = args.variant
+    f = C.go_bench_fluid(lambda *a: Fluid(*a), v)
+    cells = (f.NumX - 2) * (f.NumY - 2)
+    for _ in range(args.warmup):
+        C.go_bench_frame(f, v)
+    f.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C.go_bench_frame(f, v)
+    f.sync()
+    wall = time.perf_counter() - t0
+    value = cells * args.steps / wall
+    pub = GO_PUBLISHED_MS.get(v)
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": value / (cells / (pub * 1e-3)) if pub else None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "reference Go benchmark %s: 300x251, dt 1/120, 8 sweeps%s" % (
+            {"quiescent": "BenchmarkSimulate", "jet": "BenchmarkSimulateWithJet",
+             "bfecc": "BenchmarkSimulateBFECC"}[v], "" if v == "quiescent" else ", jet u=4 on 100 cells"),
+            "grid": [300, 251], "published_ms_per_frame": pub},
+        "roofline": None, "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        from oracle.pyoracle import OracleFluid
+        o = C.go_bench_fluid(lambda d, w, h, hh: OracleFluid(d, w, h, hh, threads=args.cpu_threads), v)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0:
+            C.go_bench_frame(o, v)
+            n += 1
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cells * n / tc, "unit": "cell-steps/s", "cores": args.cpu_threads,
+                               "kind": "port", "sample": "float32 oracle, %d frames of the same benchmark "
+                                                        "back to back, %.1f s" % (n, tc)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_ensemble(args):
     """The batched ensemble: 256 members of seeded sizes 64..1024, presets, wall strokes,
     Gaussian impulses every 10 steps, 60 sweeps, SL, stepped by one FluidBatch (one launch
@@ -268,7 +317,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble", "diff"])
+    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble", "diff", "go"])
+    ap.add_argument("--variant", default="jet", choices=["quiescent", "jet", "bfecc"],
+                    help="with --config go: which of the reference's Go benchmarks")
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--confinement", type=float, default=0.01,
                     help="differentiable rollout: vorticity confinement (0.3 diverges on the 512^2 jet)")
@@ -297,6 +348,10 @@ def main():
 
     if args.config == "diff":
         return bench_diff(args)
+    if args.config == "go":
+        if args.steps == 20:  # frames take tens of microseconds: time a few seconds' worth
+            args.steps, args.warmup = 2000, max(args.warmup, 50)
+        return bench_go(args)
     if args.config == "ensemble":
         if args.iters == 200:
             args.iters = 60

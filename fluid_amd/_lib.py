@@ -68,6 +68,7 @@ SIGNATURES = {
     "fl_set_params": (_i, [_vp, ctypes.POINTER(FlParams)]),
     "fl_upload": (_i, [_vp, _i, _vp, _sz]),
     "fl_download": (_i, [_vp, _i, _vp, _sz]),
+    "fl_fill_cells": (_i, [_vp, _i, _i, _i, _i, _i, _f]),
     "fl_simulate": (_i, [_vp, _f]),
     "fl_sync": (_i, [_vp]),
     "fl_get_stats": (_i, [_vp, ctypes.POINTER(FlStats)]),

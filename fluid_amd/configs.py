@@ -64,6 +64,34 @@ def open_all_cells(f):
     f.set("S", S)
 
 
+GO_BENCH_VARIANTS = ("quiescent", "jet", "bfecc")
+GO_BENCH_DT = 1.0 / 120.0
+
+
+def go_bench_fluid(make, variant="jet"):
+    """newBenchFluid (fluid_bench_test.go:5-17): New(1000, 300, 251, 0.01), border cells solid,
+    the interior fluid; Simulate's own 8 sweeps. The "bfecc" variant sets UseBFECC
+    (BenchmarkSimulateBFECC, fluid_bench_test.go:45-60)."""
+    f = make(1000.0, 300, 251, 0.01)
+    S = np.zeros((f.NumX, f.NumY), dtype=np.float32)
+    S[1:-1, 1:-1] = 1.0
+    f.set("S", S)
+    f.PressureIters = 8
+    f.UseBFECC = variant == "bfecc"
+    return f
+
+
+def go_bench_frame(f, variant="jet"):
+    """One benchmark iteration: BenchmarkSimulate (fluid_bench_test.go:19-27) is Simulate(1/120)
+    alone; BenchmarkSimulateWithJet and BenchmarkSimulateBFECC (:29-60) first write U = 4 and
+    M = 1 on the 100 cells i = 1, j in [NumY/2 - 50, NumY/2 + 50)."""
+    if variant != "quiescent":
+        lo = f.NumY // 2 - 50
+        f.fill_cells("U", 1, 2, lo, lo + 100, 4.0)
+        f.fill_cells("M", 1, 2, lo, lo + 100, 1.0)
+    f.Simulate(GO_BENCH_DT)
+
+
 def jet_scene(make, width=256, height=256, iters=40, jet_u=1.0, jet_rows=16, walls=(True, True, True, True)):
     """Jet: inlet at the left edge, zero initial state, all border walls."""
     f = make(DENSITY, width, height, H)

@@ -361,6 +361,13 @@ static void set_circular_obstacle(Ctx* c, int cx, int cy, int r) {
     }
 }
 
+// fl_fill_cells: one thread per column j of the range, rows strided over blockIdx.y.
+__global__ void k_fill_cells(float* __restrict__ f, int NY, int i0, int i1, int j0, int j1, float value) {
+  const int j = j0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= j1) return;
+  for (int i = i0 + blockIdx.y; i < i1; i += gridDim.y) f[(size_t)i * NY + j] = value;
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -508,6 +515,22 @@ int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n) {
     std::memcpy(c->hS.data(), src, 4 * n);
     c->flags_dirty = true;
   }
+  return FL_OK;
+}
+
+int fl_fill_cells(fl_ctx* ctx, int field, int i0, int i1, int j0, int j1, float value) {
+  CTX_CHECK(ctx);
+  Ctx* c = C_OF(ctx);
+  float* dst = field_ptr(c, field);
+  if (!dst || field == FL_FIELD_S) return err(FL_ERR_INVALID_ARG, "bad field (S changes through fl_set_solid)");
+  if (i0 < 0 || j0 < 0 || i1 > c->NX || j1 > c->NY) return err(FL_ERR_INVALID_ARG, "cell range outside the grid");
+  if (i0 >= i1 || j0 >= j1) return FL_OK;
+  int rc = flush_edits(c);
+  if (rc) return rc;
+  const int w = j1 - j0, bx = w < 256 ? ((w + 63) / 64) * 64 : 256;
+  hipLaunchKernelGGL(k_fill_cells, dim3((w + bx - 1) / bx, std::min(i1 - i0, 4096)), dim3(bx), 0, c->stream, dst,
+                     c->NY, i0, i1, j0, j1, value);
+  FL_HIP(hipGetLastError());
   return FL_OK;
 }
 

@@ -171,6 +171,11 @@ class Fluid:
         a = np.ascontiguousarray(np.asarray(arr, dtype=np.float32).reshape(self.NumX, self.NumY))
         check(self._L.fl_upload(self._ctx, FIELDS[name], a.ctypes.data, a.size))
 
+    def fill_cells(self, name, i0, i1, j0, j1, value):
+        """Direct writes name[i, j] = value for i in [i0, i1), j in [j0, j1) (any field but S),
+        on the device."""
+        check(self._L.fl_fill_cells(self._ctx, FIELDS[name], i0, i1, j0, j1, value))
+
     def state(self):
         return {k: self.get(k) for k in FIELDS}
 

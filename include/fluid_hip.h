@@ -99,6 +99,11 @@ int fl_set_params(fl_ctx* ctx, const fl_params* in);
  * (the reference exposes U,V,S,M as mutable slices, fluid.go:17-22). n = NumX*NumY. */
 int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n);
 int fl_download(fl_ctx* ctx, int field, float* dst, size_t n);
+/* A loop of direct writes f.F[i*NumY + j] = value over i in [i0, i1), j in [j0, j1) to any
+ * field but S (whose writes go through fl_set_solid), as the reference's callers do on the
+ * exported slices each frame (fluid_bench_test.go:36-39, fluid_test.go:1097-1101). Asynchronous
+ * on the context's stream, after the queued edits; an empty range does nothing. */
+int fl_fill_cells(fl_ctx* ctx, int field, int i0, int i1, int j0, int j1, float value);
 
 /* Simulate(dt) — fluid.go:79-109. Asynchronous on the context's stream. */
 int fl_simulate(fl_ctx* ctx, float dt);

@@ -241,6 +241,12 @@ class OracleFluid:
         a = np.ascontiguousarray(np.asarray(arr, dtype=self.dtype).reshape(self.NumX, self.NumY))
         self._L.orc_set_field(self._h, FIELDS[name], a.ctypes.data)
 
+    def fill_cells(self, name, i0, i1, j0, j1, value):
+        assert name != "S"
+        a = self.get(name)
+        a[i0:i1, j0:j1] = value
+        self.set(name, a)
+
     def state(self):
         return {k: self.get(k) for k in FIELDS}
 

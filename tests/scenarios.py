@@ -349,6 +349,18 @@ def bfecc_accuracy(make, use_bfecc):
     return f, {"peak smoke finite, in [0, 1]": np.isfinite(peak) and 0.0 <= peak <= 1.0}
 
 
+def go_benchmark(make, variant, frames=30):
+    """BenchmarkSimulate, BenchmarkSimulateWithJet, BenchmarkSimulateBFECC
+    (fluid_bench_test.go:19-60) run for `frames` iterations. The benchmarks assert nothing; the
+    check is that the fields stay finite."""
+    from fluid_amd import configs as C
+    f = C.go_bench_fluid(make, variant)
+    for _ in range(frames):
+        C.go_bench_frame(f, variant)
+    ok = all(np.isfinite(f.get(k)).all() for k in ("U", "V", "p", "M"))
+    return f, {"fields finite": ok}
+
+
 SCENARIOS = {
     "realistic_jet": realistic_jet,
     "obstacle_vortex_shedding": obstacle_vortex_shedding,
@@ -369,4 +381,7 @@ SCENARIOS = {
     "advanced_visual_enhancements": advanced_visual_enhancements,
     "bfecc_accuracy_sl": lambda mk: bfecc_accuracy(mk, False),
     "bfecc_accuracy_bfecc": lambda mk: bfecc_accuracy(mk, True),
+    "go_benchmark_quiescent": lambda mk: go_benchmark(mk, "quiescent"),
+    "go_benchmark_jet": lambda mk: go_benchmark(mk, "jet"),
+    "go_benchmark_bfecc": lambda mk: go_benchmark(mk, "bfecc"),
 }

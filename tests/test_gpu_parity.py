@@ -266,6 +266,28 @@ def test_fp64_per_step_error():
         assert err <= 1e-5, (k, err)
 
 
+def test_fill_cells_matches_direct_writes():
+    """fl_fill_cells against the same writes on the oracle's fields: after a queued edit, on
+    a column, a whole row of the last column and rows of V; an empty range writes nothing; S
+    and ranges outside the grid are refused."""
+    from fluid_amd._lib import FluidError
+    g, o = pair(1000.0, 37, 23, 0.01)
+    load_both(g, o, random_state(39, 25, seed=11))
+    for f in (g, o):
+        f.SetVelocity(1, 7, 1.0, 2.0)
+        f.fill_cells("U", 1, 2, 5, 15, 4.0)
+        f.fill_cells("M", 0, 39, 24, 25, 0.5)
+        f.fill_cells("V", 10, 12, 0, 25, -1.0)
+        f.fill_cells("p", 3, 3, 0, 5, 9.0)
+    assert_same(g, o, "fills")
+    g.Simulate(0.1)
+    o.Simulate(0.1)
+    assert_same(g, o, "step after fills")
+    for bad in (("S", 0, 1, 0, 1), ("U", 0, 40, 0, 1), ("U", -1, 1, 0, 1), ("M", 0, 1, 0, 26)):
+        with pytest.raises(FluidError):
+            g.fill_cells(*bad, 0.0)
+
+
 MG_SIZES = [(5, 3), (6, 4), (37, 23), (64, 64), (101, 60)]
 
 

@@ -105,6 +105,7 @@ def bench_go(args):
     from fluid_amd import configs as C
     v = args.variant
     f = C.go_bench_fluid(lambda *a: Fluid(*a), v)
+    f.Solver, f.BlockSweeps, f.HandoffLag = args.solver, args.block_sweeps, args.handoff_lag
     cells = (f.NumX - 2) * (f.NumY - 2)
     for _ in range(args.warmup):
         C.go_bench_frame(f, v)

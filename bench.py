@@ -142,6 +142,62 @@ def bench_go(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_frame(args):
+    """The per-frame outputs around the step, on the cavity state after a few steps: field
+    ranges (device reduction, two floats back), Game.Draw's image for each view (computed on
+    the device, 4 B per cell downloaded), Vorticity() (4 B per cell downloaded), and 2^20
+    tracer particles through SampleVelocityBatch and the RK2 advectParticles (particle arrays
+    up and down). Wall time per call with the host transfers; the device kernel times are in
+    the rocprof summary of the same command."""
+    import numpy as np
+    from fluid_amd import Fluid
+    from fluid_amd import configs as C
+    f = build("cavity", lambda *a: Fluid(*a), args.size, args.iters)
+    for _ in range(3):
+        f.update(C.DT)
+    f.sync()
+    W, H = f.NumX - 2, f.NumY - 2
+    n = 1 << 20
+    rng = np.random.default_rng(1)
+    px = (rng.uniform(1.5, f.NumX - 1.5, n) * f.H()).astype(np.float32)
+    py = (rng.uniform(1.5, f.NumY - 1.5, n) * f.H()).astype(np.float32)
+    age = np.zeros(n, np.float32)
+    max_age = np.full(n, 1e9, np.float32)
+    ops = {
+        "field_range_M": lambda: f.field_range("M"),
+        "field_range_p": lambda: f.field_range("p"),
+        "render_smoke": lambda: f.render(0),
+        "render_pressure": lambda: f.render(1),
+        "render_speed": lambda: f.render(2),
+        "render_vorticity": lambda: f.render(3),
+        "Vorticity": lambda: f.Vorticity(),
+        "SampleVelocityBatch_1M": lambda: f.SampleVelocityBatch(px, py),
+        "advect_particles_1M": lambda: f.advect_particles(0.01, px.copy(), py.copy(), age.copy(), max_age),
+    }
+    res = {}
+    for name, fn in ops.items():
+        for _ in range(2):
+            fn()
+        f.sync()
+        reps = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        f.sync()
+        res[name] = 1e3 * (time.perf_counter() - t0) / reps
+    frame_ms = res["render_smoke"] + res["advect_particles_1M"]
+    out = {
+        "metric": "frame outputs around the step: field ranges, rendered views, tracer particles",
+        "value": 1e3 / frame_ms, "unit": "frames/s (smoke image + 2^20 tracer particles)", "n_gpus": 1,
+        "steps": reps, "warmup": 2, "ms_per_step": frame_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cavity %dx%d after 3 steps; wall ms per call including host transfers" % (W, H),
+                   "grid": [W, H], "particles": n},
+        "ms_per_call": res, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_ensemble(args):
     """The batched ensemble: 256 members of seeded sizes 64..1024, presets, wall strokes,
     Gaussian impulses every 10 steps, 60 sweeps, SL, stepped by one FluidBatch (one launch
@@ -318,7 +374,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble", "diff", "go"])
+    ap.add_argument("--config", default="cavity", choices=["cavity", "jet", "karman", "ensemble", "diff", "go", "frame"])
     ap.add_argument("--variant", default="jet", choices=["quiescent", "jet", "bfecc"],
                     help="with --config go: which of the reference's Go benchmarks")
     ap.add_argument("--members", type=int, default=256)
@@ -349,6 +405,8 @@ def main():
 
     if args.config == "diff":
         return bench_diff(args)
+    if args.config == "frame":
+        return bench_frame(args)
     if args.config == "go":
         if args.steps == 20:  # frames take tens of microseconds: time a few seconds' worth
             args.steps, args.warmup = 2000, max(args.warmup, 50)

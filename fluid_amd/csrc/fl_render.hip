@@ -21,7 +21,7 @@
 namespace fl {
 
 static constexpr int RB = 256;          // threads per reduction block
-static constexpr int RBLOCKS = 512;     // partials of the first pass
+static constexpr int RBLOCKS = 2048;    // partials of the first pass
 
 struct MinMax {
   float mn, mx;
@@ -61,18 +61,38 @@ __device__ __forceinline__ void mm_block(MinMax m, MinMax* out) {
   }
   if (threadIdx.x == 0) *out = red[0];
 }
-__global__ __launch_bounds__(RB) void k_minmax_partial(const float* __restrict__ f, const uint8_t* __restrict__ flags,
-                                                       int interior, size_t n, MinMax* __restrict__ part) {
-  MinMax m = mm_init();
-  for (size_t k = blockIdx.x * (size_t)RB + threadIdx.x; k < n; k += (size_t)gridDim.x * RB) {
-    if (interior) {
-      const uint32_t fl = flags[k];
-      if (!((fl & CF_INTERIOR) && (fl & CF_FLUID))) continue;
-    }
-    const float v = f[k];
-    mm_take_min(m, v, (long long)k);  // NaN and values tying the initial bound are never taken
-    mm_take_max(m, v, (long long)k);
+// A thread scans its elements in increasing index order (four at a time, grid-strided, when
+// `vec` says f is 16-byte and flags 4-byte aligned; then the rest one at a time, grid-strided,
+// all above the vector part's indices), so a strict comparison keeps the lowest index among
+// equal values; indices are compared only when threads merge. NaN and values tying the
+// initial bound are never taken.
+__device__ __forceinline__ void mm_scan(MinMax& m, float v, bool take, long long k) {
+  if (take && v < m.mn) {
+    m.mn = v;
+    m.imn = k;
   }
+  if (take && v > m.mx) {
+    m.mx = v;
+    m.imx = k;
+  }
+}
+__device__ __forceinline__ bool mm_cell(uint32_t fl) { return (fl & CF_INTERIOR) && (fl & CF_FLUID); }
+__global__ __launch_bounds__(RB) void k_minmax_partial(const float* __restrict__ f, const uint8_t* __restrict__ flags,
+                                                       int interior, int vec, size_t n, MinMax* __restrict__ part) {
+  MinMax m = mm_init();
+  const size_t n4 = vec ? n / 4 : 0, stride = (size_t)gridDim.x * RB, t = blockIdx.x * (size_t)RB + threadIdx.x;
+  const float4* f4 = reinterpret_cast<const float4*>(f);
+  const uchar4* fl4 = reinterpret_cast<const uchar4*>(flags);
+  for (size_t q = t; q < n4; q += stride) {
+    const float4 v = f4[q];
+    const uchar4 fl = interior ? fl4[q] : make_uchar4(0, 0, 0, 0);
+    const long long k = 4 * (long long)q;
+    mm_scan(m, v.x, !interior || mm_cell(fl.x), k);
+    mm_scan(m, v.y, !interior || mm_cell(fl.y), k + 1);
+    mm_scan(m, v.z, !interior || mm_cell(fl.z), k + 2);
+    mm_scan(m, v.w, !interior || mm_cell(fl.w), k + 3);
+  }
+  for (size_t k = 4 * n4 + t; k < n; k += stride) mm_scan(m, f[k], !interior || mm_cell(flags[k]), (long long)k);
   mm_block(m, part + blockIdx.x);
 }
 __global__ __launch_bounds__(RB) void k_minmax_final(const MinMax* __restrict__ part, int nparts,
@@ -145,17 +165,33 @@ __device__ __forceinline__ uint32_t diverging_color(float val, float mn, float m
 
 // One pixel per thread: image (x = i, row j) shows fluid cell (i, NumY-1-j) (main.go:614-646);
 // solid cells are painted black afterwards in the reference, i.e. win (main.go:560-572).
-__global__ void k_render(int NX, int NY, const float* __restrict__ vals, const float* __restrict__ S,
-                         const float* __restrict__ range, int diverging, uint32_t* __restrict__ pix) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-  if (i >= NX || j >= NY) return;
-  const int c = i * NY + (NY - 1 - j);
-  uint32_t px;
-  if (S[c] == 0.0f)
-    px = 0xff000000u;
-  else
-    px = diverging ? diverging_color(vals[c], range[0], range[1]) : sci_color(vals[c], range[0], range[1]);
-  pix[(size_t)j * NX + i] = px;
+// Image row r shows field row j = NY-1-r, pixel i field column i: a transpose of the j-contiguous
+// field. A 64x64 block of cells is read along j, coloured, and written along i through LDS.
+static constexpr int RT = 64;
+__global__ __launch_bounds__(256) void k_render(int NX, int NY, const float* __restrict__ vals,
+                                                const float* __restrict__ S, const float* __restrict__ range,
+                                                int diverging, uint32_t* __restrict__ pix) {
+  __shared__ uint32_t tile[RT][RT + 1];
+  const int j0 = blockIdx.x * RT, i0 = blockIdx.y * RT;
+  const int lane = threadIdx.x & (RT - 1), row = threadIdx.x / RT;
+  const float mn = range[0], mx = range[1];
+  for (int r = row; r < RT; r += 256 / RT) {
+    const int i = i0 + r, j = j0 + lane;
+    if (i < NX && j < NY) {
+      const size_t c = (size_t)i * NY + j;
+      uint32_t px;
+      if (S[c] == 0.0f)
+        px = 0xff000000u;
+      else
+        px = diverging ? diverging_color(vals[c], mn, mx) : sci_color(vals[c], mn, mx);
+      tile[lane][r] = px;
+    }
+  }
+  __syncthreads();
+  for (int r = row; r < RT; r += 256 / RT) {
+    const int j = j0 + r, i = i0 + lane;
+    if (i < NX && j < NY) pix[(size_t)(NY - 1 - j) * NX + i] = tile[r][lane];
+  }
 }
 
 // Game.advectParticles (main.go:512-546) for one particle.
@@ -194,8 +230,8 @@ static int rerr(int code, const std::string& m) {
   return code;
 }
 
-// Scratch of these calls: [NX*NY floats of view values][RBLOCKS MinMax][2 floats range]
-// [pixels or particle arrays], grown on demand.
+// Scratch of these calls: [NX*NY floats of view values, padded to 256 B][RBLOCKS MinMax]
+// [2 floats range, 16 B][pixels or particle arrays], grown on demand.
 static int scratch(Ctx* c, size_t bytes, char** out) {
   if (bytes > c->rs_cap) {
     if (c->rs_mem) FL_HIP(hipFree(c->rs_mem));
@@ -208,7 +244,9 @@ static int scratch(Ctx* c, size_t bytes, char** out) {
 }
 
 static int device_minmax(Ctx* c, const float* f, int interior, MinMax* part, float* range) {
-  hipLaunchKernelGGL(k_minmax_partial, dim3(RBLOCKS), dim3(RB), 0, c->stream, f, c->flags, interior, c->N, part);
+  const int vec = (uintptr_t)f % 16 == 0 && (!interior || (uintptr_t)c->flags % 4 == 0);
+  hipLaunchKernelGGL(k_minmax_partial, dim3(RBLOCKS), dim3(RB), 0, c->stream, f, c->flags, interior, vec, c->N,
+                     part);
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(RB), 0, c->stream, part, RBLOCKS, range);
   FL_HIP(hipGetLastError());
   return FL_OK;
@@ -257,7 +295,8 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) 
   int rc = ctx_prepare(c);
   if (rc) return rc;
   char* mem;
-  const size_t o_part = 4 * c->N, o_range = o_part + sizeof(MinMax) * RBLOCKS, o_pix = o_range + 16;
+  const size_t o_part = (4 * c->N + 255) / 256 * 256, o_range = o_part + sizeof(MinMax) * RBLOCKS,
+               o_pix = o_range + 16;
   if ((rc = scratch(c, o_pix + 4 * c->N, &mem))) return rc;
   float* vals = reinterpret_cast<float*>(mem);
   float* range = reinterpret_cast<float*>(mem + o_range);
@@ -270,7 +309,7 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) 
     f = vals;
   }
   if ((rc = device_minmax(c, f, view >= 2, reinterpret_cast<MinMax*>(mem + o_part), range))) return rc;
-  hipLaunchKernelGGL(k_render, dim3((c->NX + 255) / 256, c->NY), dim3(256), 0, c->stream, c->NX, c->NY, f, c->S,
+  hipLaunchKernelGGL(k_render, dim3((c->NY + RT - 1) / RT, (c->NX + RT - 1) / RT), dim3(256), 0, c->stream, c->NX, c->NY, f, c->S,
                      range, view == 3 ? 1 : 0, pix);
   FL_HIP(hipGetLastError());
   FL_HIP(hipMemcpyAsync(rgba, pix, nbytes, hipMemcpyDeviceToHost, c->stream));

@@ -320,6 +320,12 @@ class Fluid:
         check(self._L.fl_field_minmax(self._ctx, FIELDS[name], ctypes.byref(mn), ctypes.byref(mx)))
         return ScalarField(self.NumX, self.NumY, self.get(name), mn.value, mx.value)
 
+    def field_range(self, name):
+        """(MinValue, MaxValue) of a field, reduced on the device; nothing is downloaded."""
+        mn, mx = ctypes.c_float(), ctypes.c_float()
+        check(self._L.fl_field_minmax(self._ctx, FIELDS[name], ctypes.byref(mn), ctypes.byref(mx)))
+        return mn.value, mx.value
+
     def Pressure(self):
         return self._minmax("p")
 

@@ -156,6 +156,10 @@ struct Ctx {
   char* rs_mem = nullptr;
   size_t rs_cap = 0;
 
+  // Pinned staging of large device-to-host copies (copy_to_host): two chunk buffers.
+  char* stage = nullptr;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+
   // Host mirror of S (always valid) and pending edits.
   std::vector<float> hS;
   std::vector<WordOp> edits;
@@ -267,6 +271,10 @@ struct Batch {
 
 // ---- host runtime shared by single contexts and batches (fl_capi.hip) ------------------
 int flush_edits(Ctx* c);
+// Device-to-host copy on the context's stream that returns when `dst` holds the data.
+int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes);
+// MinValue/MaxValue of a device field (all cells, or the fluid interior), reduced on the device.
+int field_range(Ctx* c, const float* f, int interior, float* mn, float* mx);
 int ctx_prepare(Ctx* c);        // flush edits, refresh flags
 int ctx_build_sources(Ctx* c);  // upload the per-step source program if it changed
 void source_ops(Ctx* c, std::vector<SrcOp>& ops);

@@ -252,6 +252,23 @@ static int device_minmax(Ctx* c, const float* f, int interior, MinMax* part, flo
   return FL_OK;
 }
 
+// The scan's MinValue/MaxValue of a device field, over every cell or (`interior`) the fluid
+// interior cells, reduced on the device; the flags must be current.
+int field_range(Ctx* c, const float* f, int interior, float* mn, float* mx) {
+  char* mem;
+  const size_t off = sizeof(MinMax) * RBLOCKS;
+  int rc = scratch(c, off + 2 * sizeof(float), &mem);
+  if (rc) return rc;
+  if ((rc = device_minmax(c, f, interior, reinterpret_cast<MinMax*>(mem), reinterpret_cast<float*>(mem + off))))
+    return rc;
+  float r[2];
+  FL_HIP(hipMemcpyAsync(r, mem + off, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  FL_HIP(hipStreamSynchronize(c->stream));
+  if (mn) *mn = r[0];
+  if (mx) *mx = r[1];
+  return FL_OK;
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -275,16 +292,7 @@ int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
   }
   int rc = ctx_prepare(c);
   if (rc) return rc;
-  char* mem;
-  const size_t off = sizeof(MinMax) * RBLOCKS;
-  if ((rc = scratch(c, off + 2 * sizeof(float), &mem))) return rc;
-  if ((rc = device_minmax(c, f, 0, reinterpret_cast<MinMax*>(mem), reinterpret_cast<float*>(mem + off)))) return rc;
-  float r[2];
-  FL_HIP(hipMemcpyAsync(r, mem + off, sizeof(r), hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipStreamSynchronize(c->stream));
-  if (mn) *mn = r[0];
-  if (mx) *mx = r[1];
-  return FL_OK;
+  return field_range(c, f, 0, mn, mx);
 }
 
 int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) {
@@ -312,8 +320,7 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) 
   hipLaunchKernelGGL(k_render, dim3((c->NY + RT - 1) / RT, (c->NX + RT - 1) / RT), dim3(256), 0, c->stream, c->NX, c->NY, f, c->S,
                      range, view == 3 ? 1 : 0, pix);
   FL_HIP(hipGetLastError());
-  FL_HIP(hipMemcpyAsync(rgba, pix, nbytes, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipStreamSynchronize(c->stream));
+  if ((rc = copy_to_host(c, rgba, pix, nbytes))) return rc;
   if (overlay) {  // source and sink markers (main.go:574-595), in the reference's order
     const long long stride = 4LL * c->NX, len = (long long)nbytes;
     auto mark = [&](const Source& s, uint8_t r, uint8_t g) {

@@ -90,6 +90,32 @@ int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
   return FL_OK;
 }
 
+// The same for host-to-device copies: the host fills one pinned buffer while the DMA of the
+// other runs; the call returns when the data is on the device.
+int copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes < STAGE_MIN) {
+    FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipStreamSynchronize(c->stream));
+    return FL_OK;
+  }
+  if (!c->stage) FL_HIP(hipHostMalloc((void**)&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
+  for (auto& e : c->stage_ev)
+    if (!e) FL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
+  for (size_t k = 0; k < nch; k++) {
+    const size_t len = std::min(STAGE_CHUNK, bytes - k * STAGE_CHUNK);
+    char* b = c->stage + (k & 1) * STAGE_CHUNK;
+    if (k >= 2) FL_HIP(hipEventSynchronize(c->stage_ev[k & 1]));  // the DMA of chunk k-2 left b
+    std::memcpy(b, s + k * STAGE_CHUNK, len);
+    FL_HIP(hipMemcpyAsync(d + k * STAGE_CHUNK, b, len, hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipEventRecord(c->stage_ev[k & 1], c->stream));
+  }
+  FL_HIP(hipStreamSynchronize(c->stream));
+  return FL_OK;
+}
+
 int flush_edits(Ctx* c) {
   if (c->edits.empty()) return FL_OK;
   const size_t n = c->edits.size();
@@ -543,8 +569,7 @@ int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n) {
   }
   int rc = flush_edits(c);
   if (rc) return rc;
-  FL_HIP(hipMemcpyAsync(dst, src, 4 * n, hipMemcpyHostToDevice, c->stream));
-  FL_HIP(hipStreamSynchronize(c->stream));
+  if ((rc = copy_to_device(c, dst, src, 4 * n))) return rc;
   if (field == FL_FIELD_S) {
     std::memcpy(c->hS.data(), src, 4 * n);
     c->flags_dirty = true;

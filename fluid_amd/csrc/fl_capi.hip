@@ -60,10 +60,12 @@ static void push_op(Ctx* c, int field, size_t idx, bool add, float v) {
 }
 
 // Apply queued edits on the device: ops are grouped per target word, order kept.
-// Copies of 16 MB and more go through two pinned 8 MB buffers: the DMA of chunk k+1 runs
-// while the host copies chunk k out, instead of the runtime's staging of pageable memory
-// (about 5 GB/s here). Smaller copies, batch members' fields included, go directly.
-static constexpr size_t STAGE_CHUNK = size_t(8) << 20, STAGE_MIN = size_t(16) << 20;
+// Copies of 2 MB and more (16 MB for a batch member, so that hundreds of members do not each
+// pin 16 MB) go through two pinned 8 MB buffers: the DMA of chunk k+1 runs while the host
+// copies chunk k out, instead of the runtime's staging of pageable memory (about 5 GB/s
+// here). Smaller copies go directly.
+static constexpr size_t STAGE_CHUNK = size_t(8) << 20;
+static size_t stage_min(const Ctx* c) { return c->batch ? size_t(16) << 20 : size_t(2) << 20; }
 // A chunk's host copy on four threads: one thread copies (and first-touches fresh pages of a
 // caller's buffer) at about 12 GB/s, below what the DMA delivers.
 static void host_copy(char* d, const char* s, size_t len) {
@@ -78,7 +80,7 @@ static void host_copy(char* d, const char* s, size_t len) {
   for (auto& x : th) x.join();
 }
 int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
-  if (bytes < STAGE_MIN) {
+  if (bytes < stage_min(c)) {
     FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     FL_HIP(hipStreamSynchronize(c->stream));
     return FL_OK;
@@ -107,7 +109,7 @@ int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
 // The same for host-to-device copies: the host fills one pinned buffer while the DMA of the
 // other runs; the call returns when the data is on the device.
 int copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes) {
-  if (bytes < STAGE_MIN) {
+  if (bytes < stage_min(c)) {
     FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     FL_HIP(hipStreamSynchronize(c->stream));
     return FL_OK;

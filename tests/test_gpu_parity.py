@@ -288,18 +288,19 @@ def test_fill_cells_matches_direct_writes():
             g.fill_cells(*bad, 0.0)
 
 
-def test_large_transfers_round_trip():
-    """Uploads and downloads of 16 MB and more go through pinned staging in 8 MB chunks: a
-    2500x1700 grid (17 MB per field, three chunks, the last one partial) round-trips bit for
+@pytest.mark.parametrize("w,h", [(1100, 700), (2500, 1700)])
+def test_large_transfers_round_trip(w, h):
+    """Uploads and downloads of 2 MB and more go through pinned staging in 8 MB chunks: one
+    partial chunk (1100x700, 3.1 MB per field) and three (2500x1700, 17 MB) round-trip bit for
     bit, and a download shows what device-side writes left."""
-    f = Fluid(1000.0, 2500, 1700, 0.01)
+    f = Fluid(1000.0, w, h, 0.01)
     a = np.random.default_rng(3).standard_normal((f.NumX, f.NumY)).astype(np.float32)
     f.set("U", a)
     assert np.array_equal(f.get("U").view(np.uint32), a.view(np.uint32))
     f.fill_cells("M", 0, f.NumX, 0, f.NumY, 0.25)
-    f.fill_cells("M", 7, 9, 3, 1000, -1.5)
+    f.fill_cells("M", 7, 9, 3, f.NumY - 1, -1.5)
     b = np.full((f.NumX, f.NumY), 0.25, np.float32)
-    b[7:9, 3:1000] = -1.5
+    b[7:9, 3:f.NumY - 1] = -1.5
     assert np.array_equal(f.get("M"), b)
     f.close()
 

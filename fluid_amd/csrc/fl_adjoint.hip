@@ -1073,8 +1073,8 @@ int fl_vjp_backward(fl_vjp* v, float* grad_fu, float* grad_fv, size_t n) {
     for (int s = s1 - 1; s >= s0; s--)
       if ((rc = adj_step(J, J->tape + (size_t)(s - s0) * NTAPE * N, J->sweeps[s - s0], J->dt))) return rc;
   }
-  FL_HIP(hipMemcpyAsync(grad_fu, J->lam[L_GFU], 4 * N, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipMemcpyAsync(grad_fv, J->lam[L_GFV], 4 * N, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = copy_to_host(c, grad_fu, J->lam[L_GFU], 4 * N)) || (rc = copy_to_host(c, grad_fv, J->lam[L_GFV], 4 * N)))
+    return rc;
   if ((rc = load_state(J, J->fin))) return rc;
   FL_HIP(hipStreamSynchronize(c->stream));
   // L_M was consumed: a second backward needs a new forward

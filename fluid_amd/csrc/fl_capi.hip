@@ -818,10 +818,9 @@ int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int 
     xy[2 * k] = xs[k];
     xy[2 * k + 1] = ys[k];
   }
-  FL_HIP(hipMemcpyAsync(c->d_xy, xy.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  if ((rc = copy_to_device(c, c->d_xy, xy.data(), 8 * (size_t)n))) return rc;
   if ((rc = launch_sample_batch(c, c->d_xy, n, c->d_xy + 2 * (size_t)n))) return rc;
-  FL_HIP(hipMemcpyAsync(uv.data(), c->d_xy + 2 * (size_t)n, 8 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipStreamSynchronize(c->stream));
+  if ((rc = copy_to_host(c, uv.data(), c->d_xy + 2 * (size_t)n, 8 * (size_t)n))) return rc;
   for (int k = 0; k < n; k++) {
     us[k] = uv[2 * k];
     vs[k] = uv[2 * k + 1];

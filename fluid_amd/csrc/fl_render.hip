@@ -354,19 +354,15 @@ int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float*
   float* da = dy + n;
   float* dm = da + n;
   uint8_t* dk = reinterpret_cast<uint8_t*>(dm + n);
-  FL_HIP(hipMemcpyAsync(dx, x, fb, hipMemcpyHostToDevice, c->stream));
-  FL_HIP(hipMemcpyAsync(dy, y, fb, hipMemcpyHostToDevice, c->stream));
-  FL_HIP(hipMemcpyAsync(da, age, fb, hipMemcpyHostToDevice, c->stream));
-  FL_HIP(hipMemcpyAsync(dm, max_age, fb, hipMemcpyHostToDevice, c->stream));
+  if ((rc = copy_to_device(c, dx, x, fb)) || (rc = copy_to_device(c, dy, y, fb)) ||
+      (rc = copy_to_device(c, da, age, fb)) || (rc = copy_to_device(c, dm, max_age, fb)))
+    return rc;
   hipLaunchKernelGGL(k_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->grid, c->U, c->V, c->S, dt, n,
                      dx, dy, da, dm, dk);
   FL_HIP(hipGetLastError());
-  FL_HIP(hipMemcpyAsync(x, dx, fb, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipMemcpyAsync(y, dy, fb, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipMemcpyAsync(age, da, fb, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipMemcpyAsync(keep, dk, (size_t)n, hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipStreamSynchronize(c->stream));
-  return FL_OK;
+  if ((rc = copy_to_host(c, x, dx, fb)) || (rc = copy_to_host(c, y, dy, fb)) || (rc = copy_to_host(c, age, da, fb)))
+    return rc;
+  return copy_to_host(c, keep, dk, (size_t)n);
 }
 
 }  // extern "C"

@@ -38,6 +38,30 @@
 
 namespace orc {
 
+// Planted mistakes. tests/test_oracle_pins.py shows that each pin of the restatement can
+// fail: it loads a second build of this file (-DORC_PLANTABLE, _build/liboracle_fluid_plant.so)
+// in which orc_set_plant(k) switches on mistake k, and checks that the pin meant for it
+// reports a mismatch. In the default build PLANT(k) is the constant false and compiles away.
+//   1 sampleField: corners (x1,y0) and (x0,y1) exchanged     2 sampleField: U/V half-cell offsets exchanged
+//   3 BFECC velocity: backward trace with -dt               4 BFECC velocity: backward trace by fwd velocity
+//   5 BFECC smoke: backward trace with -dt                  6 BFECC: 3x3 clamp skipped
+//   7 BFECC: correction orig - (bwd - orig), no 1/2         8 confinement: force sign flipped
+//   9 confinement: strength ignores the local speed        10 solveSingleGrid: omega at (iter+1)/N
+//  11 applyViscosity: dt dropped from visc                 12 advectSmoke: diffusion factor 0.3 dropped
+//  13 addTurbulence: speed |U|+|V|                         14 addTurbulence: factor min(|u|, 1)
+//  15 residual: -div + lap                                 16 restriction: corner weight 1/8
+//  17 coarse solve: relaxation 1                           18 prolongation: i and j fractions exchanged
+//  19 correction: U faces with the opposite sign           20 GetAdaptiveTimeStep: upper clamp 1x basedt
+//  21 GetAdaptiveTimeStep: speed sqrt(U^2+V^2)             22 drawLine: e2 > dy instead of e2 >= dy
+//  23 cavity preset: source row at NY-3                    24 applySources: solid cells not skipped
+//  25 Karman preset: disc centred at NX/2                  26 applyWallSettings: columns before rows
+#ifdef ORC_PLANTABLE
+static int g_plant = 0;
+#define PLANT(k) (g_plant == (k))
+#else
+#define PLANT(k) false
+#endif
+
 // ---------------------------------------------------------------------------
 // Go builtin min/max for floats (Go 1.21 spec): NaN propagates,
 // min(-0,+0) = -0, max(-0,+0) = +0.
@@ -229,7 +253,7 @@ template <class T> struct Fluid {
   void applyViscosity(T dt) {
     if (ViscosityDiffusion <= 0) return;
     const int n = NY;
-    T visc = ViscosityDiffusion * dt;
+    T visc = PLANT(11) ? ViscosityDiffusion : ViscosityDiffusion * dt;
     newU = U;
     newV = V;
     prange(threads, 1, NX - 1, [&](int i) {
@@ -267,7 +291,7 @@ template <class T> struct Fluid {
     sweepMaxDiv.clear();
     for (int iter = 0; iter < numIters; iter++) {
       if (forcedSweeps >= 0 && iter >= forcedSweeps) break;
-      T iterProgress = (T)iter / (T)numIters;
+      T iterProgress = (T)(PLANT(10) ? iter + 1 : iter) / (T)numIters;
       T relax = initialRelaxation - (initialRelaxation - minRelaxation) * iterProgress;
       T maxDiv = pressureSweep(relax, cp);
       sweepMaxDiv.push_back(maxDiv);
@@ -374,6 +398,7 @@ template <class T> struct Fluid {
       dx = h2;
       dy = h2;
     }
+    if (PLANT(2) && kind != SK_M) std::swap(dx, dy);
     int x0 = std::min(floorIndex((x - dx) * h1), NX - 1);
     T tx = ((x - dx) - (T)x0 * h) * h1;
     int x1 = std::min(x0 + 1, NX - 1);
@@ -387,6 +412,9 @@ template <class T> struct Fluid {
     }
     T sx = (T)1 - tx;
     T sy = (T)1 - ty;
+    if (PLANT(1))
+      return sx * sy * data[x0 * n + y0] + tx * sy * data[x0 * n + y1] + tx * ty * data[x1 * n + y1] +
+             sx * ty * data[x1 * n + y0];
     return sx * sy * data[x0 * n + y0] + tx * sy * data[x1 * n + y0] + tx * ty * data[x1 * n + y1] +
            sx * ty * data[x0 * n + y1];
   }
@@ -502,7 +530,7 @@ template <class T> struct Fluid {
           T y = (T)j * h + h2 - dt * v;
           T val = PreciseAdvection ? sampleCells(i, j, disp(-dt, u), disp(-dt, v), M, SK_M) : sample(x, y, M, SK_M);
           if (ViscosityDiffusion > 0) {
-            T smokeDiffusion = ViscosityDiffusion * (T)0.3f * dt;
+            T smokeDiffusion = PLANT(12) ? ViscosityDiffusion * dt : ViscosityDiffusion * (T)0.3f * dt;
             T nb = M[c - n] + M[c + n] + M[c - 1] + M[c + 1] - (T)4 * M[c];
             val += smokeDiffusion * nb;
           }
@@ -550,7 +578,12 @@ template <class T> struct Fluid {
         gy /= mag;
         T vort = curl[c];
         T localVel = std::sqrt(U[c] * U[c] + V[c] * V[c]);
-        T adaptiveStrength = Confinement * ((T)1 + localVel * (T)0.1f);
+        T adaptiveStrength = PLANT(9) ? Confinement : Confinement * ((T)1 + localVel * (T)0.1f);
+        if (PLANT(8)) {
+          U[c] -= adaptiveStrength * gy * vort * dt;
+          V[c] += adaptiveStrength * gx * vort * dt;
+          continue;
+        }
         U[c] += adaptiveStrength * gy * vort * dt;
         V[c] -= adaptiveStrength * gx * vort * dt;
       }
@@ -567,12 +600,12 @@ template <class T> struct Fluid {
         if (S[c] > 0) {
           T noiseU = (T)turb_noise(i * 137 + j * 241) * turbStrength;
           T noiseV = (T)turb_noise(i * 157 + j * 263) * turbStrength;
-          T localVel = std::sqrt(U[c] * U[c] + V[c] * V[c]);
+          T localVel = PLANT(13) ? std::fabs(U[c]) + std::fabs(V[c]) : std::sqrt(U[c] * U[c] + V[c] * V[c]);
           bool gate = localVel > (T)0.1f;
           turbGateOut[c] = gate ? 1 : 0;
           if (useTurbGateIn) gate = turbGateIn[c] != 0;
           if (gate) {
-            T factor = go_min(localVel * (T)0.5, (T)1);
+            T factor = go_min(PLANT(14) ? localVel : localVel * (T)0.5, (T)1);
             U[c] += noiseU * factor;
             V[c] += noiseV * factor;
           }
@@ -589,14 +622,14 @@ template <class T> struct Fluid {
       for (int j = 1; j < NY - 1; j++) {
         int c = i * n + j;
         if (S[c] > 0) {
-          T vel = std::fabs(U[c]) + std::fabs(V[c]);
+          T vel = PLANT(21) ? std::sqrt(U[c] * U[c] + V[c] * V[c]) : std::fabs(U[c]) + std::fabs(V[c]);
           if (vel > maxVel) maxVel = vel;
         }
       }
     if (maxVel == 0) return basedt;
     T cfl = (T)0.8f;
     T adt = cfl * h / maxVel;
-    adt = go_max(go_min(adt, basedt * (T)2), basedt * (T)0.1f);
+    adt = go_max(go_min(adt, basedt * (T)(PLANT(20) ? 1 : 2)), basedt * (T)0.1f);
     return adt;
   }
 
@@ -637,7 +670,7 @@ template <class T> struct Fluid {
         T sx0 = S[c - n], sx1 = S[c + n], sy0 = S[c - 1], sy1 = S[c + 1];
         T lap = sx0 * (p[c - n] - p[c]) + sx1 * (p[c + n] - p[c]) + sy0 * (p[c - 1] - p[c]) +
                 sy1 * (p[c + 1] - p[c]);
-        r[c] = -div - lap;
+        r[c] = PLANT(15) ? -div + lap : -div - lap;
       }
     return r;
   }
@@ -654,7 +687,7 @@ template <class T> struct Fluid {
                  (T)0.125;
           T co = (fr[(fi - 1) * NY + fj - 1] + fr[(fi + 1) * NY + fj - 1] + fr[(fi - 1) * NY + fj + 1] +
                   fr[(fi + 1) * NY + fj + 1]) *
-                 (T)0.0625;
+                 (T)(PLANT(16) ? 0.125 : 0.0625);
           out[i * cNY + j] = center + nb + co;
         }
       }
@@ -669,7 +702,7 @@ template <class T> struct Fluid {
         int fi = i * 2, fj = j * 2;
         CS[i * cNY + j] = (fi < NX && fj < NY) ? S[fi * NY + fj] : (T)0;
       }
-    T relax = (T)1.6f;
+    T relax = PLANT(17) ? (T)1 : (T)1.6f;
     for (int it = 0; it < 40; it++)
       for (int i = 1; i < cNX - 1; i++)
         for (int j = 1; j < cNY - 1; j++) {
@@ -695,6 +728,7 @@ template <class T> struct Fluid {
         int ci = i / 2, cj = j / 2;
         if (ci >= cNX - 1 || cj >= cNY - 1) continue;
         T fi = (T)(i % 2) * (T)0.5, fj = (T)(j % 2) * (T)0.5;
+        if (PLANT(18)) std::swap(fi, fj);
         corr[i * NY + j] = ((T)1 - fi) * ((T)1 - fj) * cc[ci * cNY + cj] + fi * ((T)1 - fj) * cc[(ci + 1) * cNY + cj] +
                            ((T)1 - fi) * fj * cc[ci * cNY + cj + 1] + fi * fj * cc[(ci + 1) * cNY + cj + 1];
       }
@@ -710,8 +744,13 @@ template <class T> struct Fluid {
         p[c] += corr[c] * cp;
         T sx0 = S[c - n], sx1 = S[c + n], sy0 = S[c - 1], sy1 = S[c + 1];
         T pc = corr[c];
-        U[c] -= sx0 * pc;
-        U[c + n] += sx1 * pc;
+        if (PLANT(19)) {
+          U[c] += sx0 * pc;
+          U[c + n] -= sx1 * pc;
+        } else {
+          U[c] -= sx0 * pc;
+          U[c + n] += sx1 * pc;
+        }
         V[c] -= sy0 * pc;
         V[c + 1] += sy1 * pc;
       }
@@ -719,6 +758,7 @@ template <class T> struct Fluid {
 
   // ---- BFECC (fluid.go:911-1051, 1094-1120) ----------------------------------------
   T clampToNeighbors(T val, const std::vector<T>& src, int i, int j) const {
+    if (PLANT(6)) return val;
     const int n = NY;
     T lo = src[i * n + j], hi = src[i * n + j];
     for (int di = -1; di <= 1; di++)
@@ -754,19 +794,21 @@ template <class T> struct Fluid {
         if (S[c] != 0 && S[c - n] != 0 && j < NY - 1) {
           T x = (T)i * h;
           T y = (T)j * h + h2;
-          T u = U[c];
-          T v = avgV(V, i, j);
-          x = x + dt * u;
-          y = y + dt * v;
+          T u = PLANT(4) ? fwdU[c] : U[c];
+          T v = PLANT(4) ? avgV(fwdV, i, j) : avgV(V, i, j);
+          const T st = PLANT(3) ? -dt : dt;
+          x = x + st * u;
+          y = y + st * v;
           bwdU[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdU, SK_U) : sample(x, y, fwdU, SK_U);
         }
         if (S[c] != 0 && S[c - 1] != 0 && i < NX - 1) {
           T x = (T)i * h + h2;
           T y = (T)j * h;
-          T u = avgU(U, i, j);
-          T v = V[c];
-          x = x + dt * u;
-          y = y + dt * v;
+          T u = PLANT(4) ? avgU(fwdU, i, j) : avgU(U, i, j);
+          T v = PLANT(4) ? fwdV[c] : V[c];
+          const T st = PLANT(3) ? -dt : dt;
+          x = x + st * u;
+          y = y + st * v;
           bwdV[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdV, SK_V) : sample(x, y, fwdV, SK_V);
         }
       }
@@ -775,8 +817,9 @@ template <class T> struct Fluid {
     for (int i = 1; i < NX - 1; i++)
       for (int j = 1; j < NY - 1; j++) {
         int c = i * n + j;
-        corrU[c] = origU[c] - (bwdU[c] - origU[c]) * (T)0.5;
-        corrV[c] = origV[c] - (bwdV[c] - origV[c]) * (T)0.5;
+        const T half = PLANT(7) ? (T)1 : (T)0.5;
+        corrU[c] = origU[c] - (bwdU[c] - origU[c]) * half;
+        corrV[c] = origV[c] - (bwdV[c] - origV[c]) * half;
       }
     for (int i = 1; i < NX - 1; i++)
       for (int j = 1; j < NY - 1; j++) {
@@ -806,8 +849,9 @@ template <class T> struct Fluid {
         if (S[c] != 0) {
           T u = (U[c] + U[c + n]) * (T)0.5 * SmokeAdvection;
           T v = (V[c] + V[c + 1]) * (T)0.5 * SmokeAdvection;
-          T x = (T)i * h + h2 + dt * u;
-          T y = (T)j * h + h2 + dt * v;
+          const T st = PLANT(5) ? -dt : dt;
+          T x = (T)i * h + h2 + st * u;
+          T y = (T)j * h + h2 + st * v;
           bwdM[c] = PreciseAdvection ? sampleCells(i, j, disp(dt, u), disp(dt, v), fwdM, SK_M) : sample(x, y, fwdM, SK_M);
         }
       }
@@ -816,7 +860,7 @@ template <class T> struct Fluid {
     for (int i = 1; i < NX - 1; i++)
       for (int j = 1; j < NY - 1; j++) {
         int c = i * n + j;
-        corrM[c] = origM[c] - (bwdM[c] - origM[c]) * (T)0.5;
+        corrM[c] = origM[c] - (bwdM[c] - origM[c]) * (PLANT(7) ? (T)1 : (T)0.5);
       }
     for (int i = 1; i < NX - 1; i++)
       for (int j = 1; j < NY - 1; j++) {
@@ -1003,14 +1047,20 @@ template <class T> struct Scene {
 
   // main.go:840-849 (bottom is j=0, top is j=NY-1)
   void applyWallSettings() {
+    if (PLANT(26))
+      for (int j = 0; j < f->NY; j++) {
+        f->SetSolid(0, j, wallLeft);
+        f->SetSolid(f->NX - 1, j, wallRight);
+      }
     for (int i = 0; i < f->NX; i++) {
       f->SetSolid(i, 0, wallBottom);
       f->SetSolid(i, f->NY - 1, wallTop);
     }
-    for (int j = 0; j < f->NY; j++) {
-      f->SetSolid(0, j, wallLeft);
-      f->SetSolid(f->NX - 1, j, wallRight);
-    }
+    if (!PLANT(26))
+      for (int j = 0; j < f->NY; j++) {
+        f->SetSolid(0, j, wallLeft);
+        f->SetSolid(f->NX - 1, j, wallRight);
+      }
   }
   // main.go:851-858 (also the cell initialisation of NewGame, main.go:222-227)
   void clearWalls() {
@@ -1031,11 +1081,11 @@ template <class T> struct Scene {
     } else if (preset == 1) {  // cavity
       wallTop = wallBottom = wallLeft = wallRight = true;
       jet = false;
-      for (int i = 2; i < f->NX - 2; i++) sources.push_back(Source{i, f->NY - 2, (double)lidU, 0.0});
+      for (int i = 2; i < f->NX - 2; i++) sources.push_back(Source{i, f->NY - (PLANT(23) ? 3 : 2), (double)lidU, 0.0});
     } else {  // karman
       wallTop = wallBottom = wallLeft = true;
       wallRight = false;
-      f->SetCircularObstacle(f->NX / 4, f->NY / 2, 8);
+      f->SetCircularObstacle(f->NX / (PLANT(25) ? 2 : 4), f->NY / 2, 8);
     }
     applyWallSettings();
   }
@@ -1043,7 +1093,7 @@ template <class T> struct Scene {
   void applySources() {
     for (auto& s : sources) {
       bool solid = true;
-      if (f->inGrid(s.I, s.J) && f->IsSolid(s.I, s.J, &solid) == 0 && !solid) {
+      if (f->inGrid(s.I, s.J) && f->IsSolid(s.I, s.J, &solid) == 0 && (!solid || PLANT(24))) {
         f->SetVelocity(s.I, s.J, (T)(float)s.U, (T)(float)s.V);
         f->AddSmoke(s.I, s.J, sourceSmoke);
       }
@@ -1073,7 +1123,7 @@ template <class T> struct Scene {
       if (f->inGrid(x0, y0)) f->SetSolid(x0, y0, value);
       if (x0 == x1 && y0 == y1) break;
       int e2 = 2 * err;
-      if (e2 >= dy) {
+      if (PLANT(22) ? e2 > dy : e2 >= dy) {
         err += dy;
         x0 += sx;
       }
@@ -1263,6 +1313,9 @@ struct OrcHandle {
 extern "C" {
 
 double orc_go_sin(double x) { return orc::go_sin(x); }
+#ifdef ORC_PLANTABLE
+void orc_set_plant(int k) { orc::g_plant = k; }
+#endif
 // Turbulence noise for every integer seed k in [0, n): the table checked against libm.
 void orc_noise_table(int n, float* out) {
   for (int k = 0; k < n; k++) out[k] = orc::turb_noise(k);
@@ -1525,6 +1578,69 @@ void orc_sample_velocity(void* h, float x, float y, double* u, double* v) {
     F->SampleVelocity(x, y, &uu, &vv);
     *u = uu;
     *v = vv;
+  });
+}
+// sampleField (fluid.go:357-398) of one field at (x, y) with the offsets of `kind`
+// (0 = U, 1 = V, 2 = M): the bilinear sampler on its own, for the closed-form pins.
+double orc_sample_field(void* h, int kind, int id, double x, double y) {
+  OrcHandle* H = (OrcHandle*)h;
+  if (id < 0 || id >= orc::F_COUNT || kind < 0 || kind > 2) return 0.0;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    return (double)F->sample((T)(float)x, (T)(float)y, *F->field(id), kind);
+  });
+}
+// Multigrid helpers one at a time (fluid.go:601-758, 1123-1149). Buffers have the handle's
+// precision; coarse buffers hold ((NX+1)/2) x ((NY+1)/2) values.
+void orc_mg_dims(void* h, int* cnx, int* cny) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    *cnx = (F->NX + 1) / 2;
+    *cny = (F->NY + 1) / 2;
+  });
+}
+void orc_mg_residual(void* h, void* out) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    auto r = F->computePressureResidual();
+    std::memcpy(out, r.data(), sizeof(r[0]) * r.size());
+  });
+}
+void orc_mg_restrict(void* h, const void* fine, void* coarse) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    std::vector<T> in((const T*)fine, (const T*)fine + F->numCells);
+    auto r = F->restrictResidual(in);
+    std::memcpy(coarse, r.data(), sizeof(T) * r.size());
+  });
+}
+void orc_mg_coarse_solve(void* h, const void* rhs, void* out) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    const size_t nc = (size_t)((F->NX + 1) / 2) * ((F->NY + 1) / 2);
+    std::vector<T> in((const T*)rhs, (const T*)rhs + nc);
+    auto r = F->solveCoarseGrid(in);
+    std::memcpy(out, r.data(), sizeof(T) * r.size());
+  });
+}
+void orc_mg_prolong(void* h, const void* coarse, void* fine) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    const size_t nc = (size_t)((F->NX + 1) / 2) * ((F->NY + 1) / 2);
+    std::vector<T> in((const T*)coarse, (const T*)coarse + nc);
+    auto r = F->prolongateCorrection(in);
+    std::memcpy(fine, r.data(), sizeof(T) * r.size());
+  });
+}
+void orc_mg_correct(void* h, const void* corr, double cp) {
+  OrcHandle* H = (OrcHandle*)h;
+  DISPATCH(H, {
+    using T = std::remove_reference_t<decltype(F->h)>;
+    std::vector<T> in((const T*)corr, (const T*)corr + F->numCells);
+    F->applyPressureCorrection(in, (T)(float)cp);
   });
 }
 // kind 0 = vorticity, 1 = velocity magnitude; out has the precision of the handle.

@@ -6,6 +6,7 @@ cpu_baseline leg as the checker. The product (fluid_amd) never imports it.
 `OracleFluid` exposes the same method names as fluid_amd.Fluid (which mirror the Go
 API of pkg/fluid), so scene builders and tests drive either object the same way.
 """
+import contextlib
 import ctypes
 import os
 import subprocess
@@ -14,6 +15,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle_fluid.so")
+PLANT_LIB_PATH = os.path.join(HERE, "_build", "liboracle_fluid_plant.so")
 
 FIELDS = {"U": 0, "V": 1, "newU": 2, "newV": 3, "p": 4, "S": 5, "M": 6, "newM": 7}
 PASSES = {
@@ -37,73 +39,104 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _load(path):
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    _bind(L)
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
-        vp, i, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
-        dp = ctypes.POINTER(ctypes.c_double)
-        ip = ctypes.POINTER(ctypes.c_int)
-        sig = {
-            "orc_go_sin": (d, [d]),
-            "orc_turb_noise": (f, [i]),
-            "orc_noise_table": (None, [i, vp]),
-            "orc_go_sin_many": (None, [i, vp, vp]),
-            "orc_new": (vp, [i, f, i, i, f]),
-            "orc_free": (None, [vp]),
-            "orc_nx": (i, [vp]),
-            "orc_ny": (i, [vp]),
-            "orc_set_field": (i, [vp, i, vp]),
-            "orc_get_field": (i, [vp, i, vp]),
-            "orc_set_params": (None, [vp, dp, ip]),
-            "orc_set_threads": (None, [vp, i]),
-            "orc_simulate": (None, [vp, f]),
-            "orc_pass": (i, [vp, i, f, i]),
-            "orc_last_sweeps": (i, [vp]),
-            "orc_sweep_maxdiv": (i, [vp, dp, i]),
-            "orc_invalid_index": (i, [vp]),
-            "orc_set_forced_sweeps": (None, [vp, i]),
-            "orc_get_turb_gate": (None, [vp, vp]),
-            "orc_set_turb_gate": (None, [vp, vp]),
-            "orc_set_solid": (i, [vp, i, i, i]),
-            "orc_is_solid": (i, [vp, i, i, ip]),
-            "orc_set_velocity": (i, [vp, i, i, d, d]),
-            "orc_add_smoke": (i, [vp, i, i, d]),
-            "orc_reset": (None, [vp]),
-            "orc_apply_force": (None, [vp, i, i, d, d]),
-            "orc_apply_force_radius": (None, [vp, i, i, d, d, i]),
-            "orc_apply_force_gaussian": (None, [vp, i, i, d, d, d]),
-            "orc_set_circular_obstacle": (None, [vp, i, i, i]),
-            "orc_adaptive_dt": (d, [vp, f]),
-            "orc_max_divergence": (d, [vp]),
-            "orc_sample_velocity": (None, [vp, f, f, dp, dp]),
-            "orc_view": (None, [vp, i, vp, dp, dp]),
-            "orc_field_minmax": (None, [vp, i, dp, dp]),
-            "orc_scene_walls": (None, [vp, i, i, i, i]),
-            "orc_scene_apply_walls": (None, [vp]),
-            "orc_scene_clear_walls": (None, [vp]),
-            "orc_scene_preset": (None, [vp, i, f]),
-            "orc_scene_jet": (None, [vp, i, i, i, i, f, f]),
-            "orc_scene_jet_on": (i, [vp]),
-            "orc_scene_add_source": (None, [vp, i, i, i, f, f]),
-            "orc_scene_clear_sources": (None, [vp]),
-            "orc_scene_num_sources": (i, [vp, i]),
-            "orc_scene_apply_sources": (None, [vp]),
-            "orc_scene_update": (None, [vp, f]),
-            "orc_scene_draw_line": (None, [vp, i, i, i, i, i]),
-            "orc_scene_smoke_brush": (None, [vp, i, i, i, f]),
-            "orc_rollout": (d, [vp, i, f, vp, vp, vp, vp, vp]),
-            "orc_advect_particles": (None, [vp, f, i, vp, vp, vp, vp, vp]),
-            "orc_render": (None, [vp, i, i, vp]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def planted(k):
+    """Run the block on the plantable build of the oracle with planted mistake k switched on
+    (the list is at the top of fluid_oracle.cpp). Objects made inside use that build."""
+    global _lib
+    L = _load(PLANT_LIB_PATH)
+    L.orc_set_plant.argtypes = [ctypes.c_int]
+    L.orc_set_plant(int(k))
+    saved, _lib = _lib, L
+    try:
+        yield
+    finally:
+        L.orc_set_plant(0)
+        _lib = saved
+
+
+def _bind(L):
+    vp, i, f, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_double
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "orc_go_sin": (d, [d]),
+        "orc_turb_noise": (f, [i]),
+        "orc_noise_table": (None, [i, vp]),
+        "orc_go_sin_many": (None, [i, vp, vp]),
+        "orc_new": (vp, [i, f, i, i, f]),
+        "orc_free": (None, [vp]),
+        "orc_nx": (i, [vp]),
+        "orc_ny": (i, [vp]),
+        "orc_set_field": (i, [vp, i, vp]),
+        "orc_get_field": (i, [vp, i, vp]),
+        "orc_set_params": (None, [vp, dp, ip]),
+        "orc_set_threads": (None, [vp, i]),
+        "orc_simulate": (None, [vp, f]),
+        "orc_pass": (i, [vp, i, f, i]),
+        "orc_last_sweeps": (i, [vp]),
+        "orc_sweep_maxdiv": (i, [vp, dp, i]),
+        "orc_invalid_index": (i, [vp]),
+        "orc_set_forced_sweeps": (None, [vp, i]),
+        "orc_get_turb_gate": (None, [vp, vp]),
+        "orc_set_turb_gate": (None, [vp, vp]),
+        "orc_set_solid": (i, [vp, i, i, i]),
+        "orc_is_solid": (i, [vp, i, i, ip]),
+        "orc_set_velocity": (i, [vp, i, i, d, d]),
+        "orc_add_smoke": (i, [vp, i, i, d]),
+        "orc_reset": (None, [vp]),
+        "orc_apply_force": (None, [vp, i, i, d, d]),
+        "orc_apply_force_radius": (None, [vp, i, i, d, d, i]),
+        "orc_apply_force_gaussian": (None, [vp, i, i, d, d, d]),
+        "orc_set_circular_obstacle": (None, [vp, i, i, i]),
+        "orc_adaptive_dt": (d, [vp, f]),
+        "orc_max_divergence": (d, [vp]),
+        "orc_sample_velocity": (None, [vp, f, f, dp, dp]),
+        "orc_view": (None, [vp, i, vp, dp, dp]),
+        "orc_field_minmax": (None, [vp, i, dp, dp]),
+        "orc_scene_walls": (None, [vp, i, i, i, i]),
+        "orc_scene_apply_walls": (None, [vp]),
+        "orc_scene_clear_walls": (None, [vp]),
+        "orc_scene_preset": (None, [vp, i, f]),
+        "orc_scene_jet": (None, [vp, i, i, i, i, f, f]),
+        "orc_scene_jet_on": (i, [vp]),
+        "orc_scene_add_source": (None, [vp, i, i, i, f, f]),
+        "orc_scene_clear_sources": (None, [vp]),
+        "orc_scene_num_sources": (i, [vp, i]),
+        "orc_scene_apply_sources": (None, [vp]),
+        "orc_scene_update": (None, [vp, f]),
+        "orc_scene_draw_line": (None, [vp, i, i, i, i, i]),
+        "orc_scene_smoke_brush": (None, [vp, i, i, i, f]),
+        "orc_rollout": (d, [vp, i, f, vp, vp, vp, vp, vp]),
+        "orc_advect_particles": (None, [vp, f, i, vp, vp, vp, vp, vp]),
+        "orc_render": (None, [vp, i, i, vp]),
+        "orc_sample_field": (d, [vp, i, i, d, d]),
+        "orc_mg_dims": (None, [vp, ip, ip]),
+        "orc_mg_residual": (None, [vp, vp]),
+        "orc_mg_restrict": (None, [vp, vp, vp]),
+        "orc_mg_coarse_solve": (None, [vp, vp, vp]),
+        "orc_mg_prolong": (None, [vp, vp, vp]),
+        "orc_mg_correct": (None, [vp, vp, d]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
 
 
 def go_sin(x):
@@ -326,6 +359,50 @@ class OracleFluid:
 
     def SetCircularObstacle(self, cx, cy, r):
         self._L.orc_set_circular_obstacle(self._h, cx, cy, r)
+
+    # ---- single functions for the closed-form pins ----
+    def sample_field(self, kind, name, x, y):
+        """sampleField (fluid.go:357-398) of field `name` at (x, y); kind "U", "V" or "M"
+        selects the half-cell offsets."""
+        return self._L.orc_sample_field(self._h, {"U": 0, "V": 1, "M": 2}[kind], FIELDS[name], float(x),
+                                        float(y))
+
+    def mg_dims(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._L.orc_mg_dims(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def mg_residual(self):
+        """computePressureResidual (fluid.go:601-633) of the current state."""
+        out = np.empty((self.NumX, self.NumY), dtype=self.dtype)
+        self._L.orc_mg_residual(self._h, out.ctypes.data)
+        return out
+
+    def mg_restrict(self, fine):
+        """restrictResidual (fluid.go:634-665)."""
+        fine = np.ascontiguousarray(np.asarray(fine, dtype=self.dtype).reshape(self.NumX, self.NumY))
+        out = np.empty(self.mg_dims(), dtype=self.dtype)
+        self._L.orc_mg_restrict(self._h, fine.ctypes.data, out.ctypes.data)
+        return out
+
+    def mg_coarse_solve(self, rhs):
+        """solveCoarseGrid (fluid.go:667-726): 40 GS sweeps at 1.6 on the coarse mask."""
+        rhs = np.ascontiguousarray(np.asarray(rhs, dtype=self.dtype).reshape(self.mg_dims()))
+        out = np.empty(self.mg_dims(), dtype=self.dtype)
+        self._L.orc_mg_coarse_solve(self._h, rhs.ctypes.data, out.ctypes.data)
+        return out
+
+    def mg_prolong(self, coarse):
+        """prolongateCorrection (fluid.go:728-758)."""
+        coarse = np.ascontiguousarray(np.asarray(coarse, dtype=self.dtype).reshape(self.mg_dims()))
+        out = np.empty((self.NumX, self.NumY), dtype=self.dtype)
+        self._L.orc_mg_prolong(self._h, coarse.ctypes.data, out.ctypes.data)
+        return out
+
+    def mg_correct(self, corr, cp):
+        """applePressureCorrection (fluid.go:1123-1149), in place on p, U, V."""
+        corr = np.ascontiguousarray(np.asarray(corr, dtype=self.dtype).reshape(self.NumX, self.NumY))
+        self._L.orc_mg_correct(self._h, corr.ctypes.data, float(cp))
 
     # ---- views ----
     def GetAdaptiveTimeStep(self, basedt):

@@ -269,11 +269,16 @@ class Fluid:
 
     def advect_particles(self, dt, x, y, age, max_age):
         """Game.advectParticles (main.go:512-546) on float32 arrays, in place; returns keep."""
-        for a in (x, y, age, max_age):
-            assert a.dtype == np.float32 and a.flags.c_contiguous and a.size == x.size
+        arrs = (x, y, age, max_age)
+        for a in arrs:
+            if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous:
+                raise TypeError("particle arrays must be C-contiguous float32 numpy arrays")
+            if a.size != x.size:
+                raise ValueError("particle arrays must have the same length")
         keep = np.zeros(x.size, dtype=np.uint8)
-        check(self._L.fl_advect_particles(self._ctx, dt, x.size, x.ctypes.data, y.ctypes.data, age.ctypes.data,
-                                          max_age.ctypes.data, keep.ctypes.data))
+        fn = self._L.fl_advect_particles
+        check(fn(self._ctx, dt, x.size, x.ctypes.data, y.ctypes.data, age.ctypes.data, max_age.ctypes.data,
+                 keep.ctypes.data))
         return keep
 
     def SetCircularObstacle(self, cx, cy, radius):
@@ -296,8 +301,10 @@ class Fluid:
         return u.value, v.value
 
     def SampleVelocityBatch(self, xs, ys):
-        xs = np.ascontiguousarray(xs, dtype=np.float32)
-        ys = np.ascontiguousarray(ys, dtype=np.float32)
+        xs = np.ascontiguousarray(xs, dtype=np.float32).reshape(-1)
+        ys = np.ascontiguousarray(ys, dtype=np.float32).reshape(-1)
+        if xs.size != ys.size:
+            raise ValueError("xs and ys must have the same length (%d != %d)" % (xs.size, ys.size))
         us, vs = np.empty_like(xs), np.empty_like(ys)
         check(self._L.fl_sample_velocity_batch(self._ctx, xs.ctypes.data, ys.ctypes.data, xs.size, us.ctypes.data,
                                                vs.ctypes.data))

@@ -10,6 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FLUID_AMD_LIB", os.path.join(HERE, "libfluidhip.so"))
 
 FL_OK = 0
+PARTICLES_NOALLOC_MAX = 131072  # FL_PARTICLES_NOALLOC_MAX
 STATUS = {0: "FL_OK", 1: "FL_ERR_INVALID_ARG", 2: "FL_ERR_X_RANGE", 3: "FL_ERR_Y_RANGE", 4: "FL_ERR_NO_DEVICE",
           5: "FL_ERR_HIP", 6: "FL_ERR_UNSUPPORTED", 7: "FL_ERR_INTERNAL"}
 
@@ -105,6 +106,7 @@ SIGNATURES = {
     "fl_apply_force_gaussian": (_i, [_vp, _i, _i, _f, _f, _f]),
     "fl_render": (_i, [_vp, _i, _i, _vp, _sz]),
     "fl_advect_particles": (_i, [_vp, _f, _i, _vp, _vp, _vp, _vp, _vp]),
+    "fl_advect_particles_noalloc": (_i, [_vp, _f, _i, _vp, _vp, _vp, _vp, _vp]),
     "fl_autotune": (_i, [_vp, _f, _ip, ctypes.POINTER(_f)]),
     "fl_batch_create": (_i, [_i, _ip, _ip, _f, _f, ctypes.POINTER(_vp)]),
     "fl_batch_destroy": (_i, [_vp]),

@@ -66,6 +66,7 @@ static void push_op(Ctx* c, int field, size_t idx, bool add, float v) {
 // here). Smaller copies go directly.
 static constexpr size_t STAGE_CHUNK = size_t(8) << 20;
 static size_t stage_min(const Ctx* c) { return c->batch ? size_t(16) << 20 : size_t(2) << 20; }
+bool copy_is_direct(const Ctx* c, size_t bytes) { return bytes < stage_min(c); }
 // A chunk's host copy on four threads: one thread copies (and first-touches fresh pages of a
 // caller's buffer) at about 12 GB/s, below what the DMA delivers.
 static void host_copy(char* d, const char* s, size_t len) {

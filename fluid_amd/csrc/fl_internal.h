@@ -274,6 +274,9 @@ int flush_edits(Ctx* c);
 // Device-to-host copy on the context's stream that returns when `dst` holds the data.
 int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes);
 int copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes);
+// True when copy_to_host / copy_to_device move `bytes` with one plain async copy (no pinned
+// staging buffer, which those calls allocate on first use).
+bool copy_is_direct(const Ctx* c, size_t bytes);
 // MinValue/MaxValue of a device field (all cells, or the fluid interior), reduced on the device.
 int field_range(Ctx* c, const float* f, int interior, float* mn, float* mx);
 int ctx_prepare(Ctx* c);        // flush edits, refresh flags

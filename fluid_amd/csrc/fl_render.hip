@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -232,7 +233,7 @@ static int rerr(int code, const std::string& m) {
 
 // Scratch of these calls: [NX*NY floats of view values, padded to 256 B][RBLOCKS MinMax]
 // [2 floats range, 16 B][pixels or particle arrays], grown on demand.
-static int scratch(Ctx* c, size_t bytes, char** out) {
+static int grow_scratch(Ctx* c, size_t bytes, char** out) {
   if (bytes > c->rs_cap) {
     if (c->rs_mem) FL_HIP(hipFree(c->rs_mem));
     c->rs_mem = nullptr;
@@ -241,6 +242,72 @@ static int scratch(Ctx* c, size_t bytes, char** out) {
   }
   *out = c->rs_mem;
   return FL_OK;
+}
+
+// Static scratch of fl_advect_particles_noalloc, for callers that cannot allocate: the
+// particles' x, y, age and max_age, FL_PARTICLES_NOALLOC_MAX floats each, then their keep
+// bytes. Each GPU holds its own copy for the life of the process; every call on a GPU uses
+// it, so overlapping calls on one GPU (two contexts, two threads) race. Nothing in it must
+// start at zero: every call copies all inputs in and k_particles writes every keep byte.
+static __device__ __align__(256) char scratch[17 * (size_t)FL_PARTICLES_NOALLOC_MAX];
+static constexpr int MAX_DEVICES = 64;
+static char* g_scratch_ptr[MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on device `dev`, looked up once per device with that device
+// current (the caller has made it current). The array's name is a host-side stand-in and is
+// never passed to a kernel or a copy.
+static int particle_scratch(int dev, char** out) {
+  if (dev < 0 || dev >= MAX_DEVICES) return rerr(FL_ERR_INTERNAL, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return rerr(FL_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FL_OK;
+}
+
+// Game.advectParticles for n particles whose device arrays are carved from `mem` (17 n
+// bytes: x, y, age, max_age as float [n] each, keep as uint8 [n]). Copies below the staging
+// threshold are issued asynchronously with one synchronisation at the end; larger ones go
+// through the pinned staging of copy_to_device / copy_to_host.
+static int advect_particles_in(Ctx* c, float dt, int n, float* x, float* y, float* age, const float* max_age,
+                               uint8_t* keep, char* mem) {
+  const size_t fb = 4 * (size_t)n;
+  float* dx = reinterpret_cast<float*>(mem);
+  float* dy = dx + n;
+  float* da = dy + n;
+  float* dm = da + n;
+  uint8_t* dk = reinterpret_cast<uint8_t*>(dm + n);
+  const bool direct = copy_is_direct(c, fb);
+  int rc;
+  if (direct) {
+    FL_HIP(hipMemcpyAsync(dx, x, fb, hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipMemcpyAsync(dy, y, fb, hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipMemcpyAsync(da, age, fb, hipMemcpyHostToDevice, c->stream));
+    FL_HIP(hipMemcpyAsync(dm, max_age, fb, hipMemcpyHostToDevice, c->stream));
+  } else if ((rc = copy_to_device(c, dx, x, fb)) || (rc = copy_to_device(c, dy, y, fb)) ||
+             (rc = copy_to_device(c, da, age, fb)) || (rc = copy_to_device(c, dm, max_age, fb))) {
+    return rc;
+  }
+  hipLaunchKernelGGL(k_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->grid, c->U, c->V, c->S, dt, n,
+                     dx, dy, da, dm, dk);
+  FL_HIP(hipGetLastError());
+  if (direct) {
+    FL_HIP(hipMemcpyAsync(x, dx, fb, hipMemcpyDeviceToHost, c->stream));
+    FL_HIP(hipMemcpyAsync(y, dy, fb, hipMemcpyDeviceToHost, c->stream));
+    FL_HIP(hipMemcpyAsync(age, da, fb, hipMemcpyDeviceToHost, c->stream));
+    FL_HIP(hipMemcpyAsync(keep, dk, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    FL_HIP(hipStreamSynchronize(c->stream));
+    return FL_OK;
+  }
+  if ((rc = copy_to_host(c, x, dx, fb)) || (rc = copy_to_host(c, y, dy, fb)) || (rc = copy_to_host(c, age, da, fb)))
+    return rc;
+  return copy_to_host(c, keep, dk, (size_t)n);
 }
 
 static int device_minmax(Ctx* c, const float* f, int interior, MinMax* part, float* range) {
@@ -257,7 +324,7 @@ static int device_minmax(Ctx* c, const float* f, int interior, MinMax* part, flo
 int field_range(Ctx* c, const float* f, int interior, float* mn, float* mx) {
   char* mem;
   const size_t off = sizeof(MinMax) * RBLOCKS;
-  int rc = scratch(c, off + 2 * sizeof(float), &mem);
+  int rc = grow_scratch(c, off + 2 * sizeof(float), &mem);
   if (rc) return rc;
   if ((rc = device_minmax(c, f, interior, reinterpret_cast<MinMax*>(mem), reinterpret_cast<float*>(mem + off))))
     return rc;
@@ -305,7 +372,7 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) 
   char* mem;
   const size_t o_part = (4 * c->N + 255) / 256 * 256, o_range = o_part + sizeof(MinMax) * RBLOCKS,
                o_pix = o_range + 16;
-  if ((rc = scratch(c, o_pix + 4 * c->N, &mem))) return rc;
+  if ((rc = grow_scratch(c, o_pix + 4 * c->N, &mem))) return rc;
   float* vals = reinterpret_cast<float*>(mem);
   float* range = reinterpret_cast<float*>(mem + o_range);
   uint32_t* pix = reinterpret_cast<uint32_t*>(mem + o_pix);
@@ -344,25 +411,35 @@ int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float*
   if (n < 0 || (n > 0 && (!x || !y || !age || !max_age || !keep))) return rerr(FL_ERR_INVALID_ARG, "bad arguments");
   if (n == 0) return FL_OK;
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (hipSetDevice(c->dev) != hipSuccess) return rerr(FL_ERR_HIP, "hipSetDevice");
   int rc = ctx_prepare(c);
   if (rc) return rc;
   char* mem;
-  const size_t fb = 4 * (size_t)n;
-  if ((rc = scratch(c, 4 * fb + (size_t)n, &mem))) return rc;
-  float* dx = reinterpret_cast<float*>(mem);
-  float* dy = dx + n;
-  float* da = dy + n;
-  float* dm = da + n;
-  uint8_t* dk = reinterpret_cast<uint8_t*>(dm + n);
-  if ((rc = copy_to_device(c, dx, x, fb)) || (rc = copy_to_device(c, dy, y, fb)) ||
-      (rc = copy_to_device(c, da, age, fb)) || (rc = copy_to_device(c, dm, max_age, fb)))
-    return rc;
-  hipLaunchKernelGGL(k_particles, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->grid, c->U, c->V, c->S, dt, n,
-                     dx, dy, da, dm, dk);
-  FL_HIP(hipGetLastError());
-  if ((rc = copy_to_host(c, x, dx, fb)) || (rc = copy_to_host(c, y, dy, fb)) || (rc = copy_to_host(c, age, da, fb)))
-    return rc;
-  return copy_to_host(c, keep, dk, (size_t)n);
+  if ((rc = grow_scratch(c, 17 * (size_t)n, &mem))) return rc;
+  return advect_particles_in(c, dt, n, x, y, age, max_age, keep, mem);
+}
+
+int fl_advect_particles_noalloc(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age,
+                                const float* max_age, uint8_t* keep) {
+  if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
+  if (n < 0 || (n > 0 && (!x || !y || !age || !max_age || !keep))) return rerr(FL_ERR_INVALID_ARG, "bad arguments");
+  if (n > FL_PARTICLES_NOALLOC_MAX)
+    return rerr(FL_ERR_INVALID_ARG, "fl_advect_particles_noalloc: n = " + std::to_string(n) +
+                                        " exceeds the static scratch (" + std::to_string(FL_PARTICLES_NOALLOC_MAX) +
+                                        " particles); use fl_advect_particles");
+  if (n == 0) return FL_OK;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (hipSetDevice(c->dev) != hipSuccess) return rerr(FL_ERR_HIP, "hipSetDevice");
+  // Queued edits are flushed through the context's edit buffers, which grow by allocation.
+  if (c->edits.size() > c->edit_cap)
+    return rerr(FL_ERR_INVALID_ARG, "fl_advect_particles_noalloc: " + std::to_string(c->edits.size()) +
+                                        " queued edits exceed the edit buffer (" + std::to_string(c->edit_cap) +
+                                        "); flush them with a step or fl_sync first");
+  char* mem;
+  int rc = particle_scratch(c->dev, &mem);
+  if (rc) return rc;
+  if ((rc = ctx_prepare(c))) return rc;
+  return advect_particles_in(c, dt, n, x, y, age, max_age, keep, mem);
 }
 
 }  // extern "C"

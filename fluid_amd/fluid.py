@@ -267,8 +267,10 @@ class Fluid:
         check(self._L.fl_render(self._ctx, int(view), int(bool(overlay)), pix.ctypes.data, pix.size))
         return pix
 
-    def advect_particles(self, dt, x, y, age, max_age):
-        """Game.advectParticles (main.go:512-546) on float32 arrays, in place; returns keep."""
+    def advect_particles(self, dt, x, y, age, max_age, no_alloc=False):
+        """Game.advectParticles (main.go:512-546) on float32 arrays, in place; returns keep.
+        no_alloc=True stages the particles in the library's static device array
+        (fl_advect_particles_noalloc: at most PARTICLES_NOALLOC_MAX particles, no allocation)."""
         arrs = (x, y, age, max_age)
         for a in arrs:
             if not isinstance(a, np.ndarray) or a.dtype != np.float32 or not a.flags.c_contiguous:
@@ -276,7 +278,7 @@ class Fluid:
             if a.size != x.size:
                 raise ValueError("particle arrays must have the same length")
         keep = np.zeros(x.size, dtype=np.uint8)
-        fn = self._L.fl_advect_particles
+        fn = self._L.fl_advect_particles_noalloc if no_alloc else self._L.fl_advect_particles
         check(fn(self._ctx, dt, x.size, x.ctypes.data, y.ctypes.data, age.ctypes.data, max_age.ctypes.data,
                  keep.ctypes.data))
         return keep

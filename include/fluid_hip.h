@@ -164,6 +164,17 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes);
  * updated in place; the caller drops the particles with keep = 0 (keeping the order). */
 int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age, const float* max_age,
                         uint8_t* keep);
+/* fl_advect_particles for callers that cannot allocate device memory (code under HIP graph
+ * capture, per-frame loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * The particle arrays are staged in a static device array of FL_PARTICLES_NOALLOC_MAX
+ * particles (2.2 MB, held on each GPU used until the process ends) instead of the context's
+ * scratch, which fl_advect_particles grows with hipMalloc. Same results as
+ * fl_advect_particles. n > FL_PARTICLES_NOALLOC_MAX returns FL_ERR_INVALID_ARG, and so do
+ * queued edits too many for the context's edit buffer (flush them with a step first). All
+ * calls on one GPU share the array: calls that overlap on one GPU race. */
+#define FL_PARTICLES_NOALLOC_MAX 131072
+int fl_advect_particles_noalloc(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age,
+                                const float* max_age, uint8_t* keep);
 
 /* Unexported passes, for white-box tests in the manner of fluid_test.go. */
 typedef enum fl_pass_id {

@@ -32,6 +32,7 @@ class FlParams(ctypes.Structure):
         ("block_sweeps", ctypes.c_int32),
         ("precise_advection", ctypes.c_int32),
         ("handoff_lag", ctypes.c_int32),
+        ("block_cols", ctypes.c_int32),
     ]
 
 

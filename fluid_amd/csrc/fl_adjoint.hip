@@ -924,6 +924,7 @@ static int adj_step(Vjp* J, const float* tp, int K, float dt) {
     m->prm.pressure_damping = P.pressure_damping;
     m->prm.block_sweeps = P.block_sweeps;
     m->prm.handoff_lag = P.handoff_lag;
+    m->prm.block_cols = P.block_cols;
     if ((rc = gs_wave_solve(m, K, 1.0f))) return rc;
     hipLaunchKernelGGL(k_mirror_out, gr, bl, 0, st, g, c->S, m->U, m->V, m->p, L[L_OU], L[L_OV]);
     FL_HIP(hipGetLastError());

@@ -171,9 +171,9 @@ static int prepare_batch(Batch* B, float dt) {
   }
   // the batch's own tile layout replaces whatever GS workspace a member holds from
   // stepping on its own
-  const int old_n = B->gs_n, old_tk = B->tk;
+  const int old_n = B->gs_n, old_tk = B->tk, old_blk = B->blk, old_grid = B->grid;
   if (N > 0 && (rc = batch_gs_layout(B, N, d))) return rc;
-  if (N > 0 && (B->gs_n != old_n || B->tk != old_tk)) B->graph_gen++;
+  if (N > 0 && (B->gs_n != old_n || B->tk != old_tk || B->blk != old_blk || B->grid != old_grid)) B->graph_gen++;
   if (B->descs.size() != d.size() || std::memcmp(B->descs.data(), d.data(), sizeof(SimDesc) * d.size()) != 0) {
     FL_HIP(hipStreamSynchronize(B->stream));  // earlier steps may still read the descriptors
     FL_HIP(hipMemcpy(B->d_sims, d.data(), sizeof(SimDesc) * d.size(), hipMemcpyHostToDevice));
@@ -278,6 +278,31 @@ static int batch_update(Batch* B, float dt) {
 }
 
 // ---- autotuning of the Gauss-Seidel tile depth ---------------------------------------------
+// Tile shapes the autotuners time: sweeps per tile (2, 4, 8, 16), columns per block (4, 8,
+// 16) and handoff publication (1 on time, 2 a block late, 3 halfway), ordered so that the
+// layout changes only with the first two.
+// Shallow tiles multiply the handoff traffic and the tile count, so 2-deep tiles are tried
+// only where the 8-deep tiling has at most two tiles per CU (the critical path, not the CUs'
+// throughput, bounds such solves).
+struct TuneShape {
+  int tk, blk, lag;
+};
+static std::vector<TuneShape> tune_shapes(long long tiles8, int cus) {
+  std::vector<TuneShape> v;
+  for (int tk : {2, 4, 8, 16}) {
+    if (tk == 2 && tiles8 > 2LL * cus) continue;
+    for (int blk : {4, 8, 16})
+      for (int lag : {1, 2, 3}) v.push_back(TuneShape{tk, blk, lag});
+  }
+  return v;
+}
+static long long tiles_at(int NX, int N, int tk) { return (long long)((NX + N - 1 + 63) / 64) * ((N + tk - 1) / tk); }
+static int device_cus(int dev) {
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+  return n;
+}
+
 // The solve reads U, V, p and writes only scratch outputs (U2, V2, T3) and its statistics,
 // so candidates are timed on the live state without changing it.
 static float time_ms(hipStream_t s, const std::function<int()>& fn, int reps, int* rc) {
@@ -460,25 +485,35 @@ int fl_batch_set_graph(fl_batch* b, int on) {
 int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
   if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
   Batch* B = B_OF(b);
-  if (B->prm.pressure_iters <= 0) return FL_OK;
-  int best = 0, rc = 0;
+  const int N = B->prm.pressure_iters;
+  if (N <= 0) return FL_OK;
+  if (hipSetDevice(B->dev) != hipSuccess) return berr(FL_ERR_HIP, "hipSetDevice");
+  long long tiles8 = 0;
+  for (Ctx* c : B->members) tiles8 += tiles_at(c->NX, N, 8);
+  TuneShape best{};
   float best_ms = 1e30f;
+  int rc = 0;
   const fl_params saved = B->prm;
-  for (int tk : {8, 16}) {
-    B->prm.block_sweeps = tk;
-    if ((rc = prepare_batch(B, dt))) return rc;
+  for (const TuneShape& sh : tune_shapes(tiles8, device_cus(B->dev))) {
+    B->prm.block_sweeps = sh.tk;
+    B->prm.block_cols = sh.blk;
+    B->prm.handoff_lag = sh.lag;
+    if ((rc = prepare_batch(B, dt))) break;
     // trial solves write only scratch roles (U2, V2, T2), so the members' state is unchanged
-    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, B->prm.pressure_iters, -1, B_T2); }, 2, &rc);
-    if (rc) return rc;
+    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, N, -1, B_T2); }, 2, &rc);
+    if (rc) break;
     if (t < best_ms) {
       best_ms = t;
-      best = tk;
+      best = sh;
     }
   }
   B->prm = saved;
-  B->prm.block_sweeps = best;
+  if (rc) return rc;
+  B->prm.block_sweeps = best.tk;
+  B->prm.block_cols = best.blk;
+  B->prm.handoff_lag = best.lag;
   if ((rc = prepare_batch(B, dt))) return rc;
-  if (block_sweeps) *block_sweeps = best;
+  if (block_sweeps) *block_sweeps = best.tk;
   if (ms) *ms = best_ms;
   return FL_OK;
 }
@@ -488,15 +523,18 @@ int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   const int N = c->prm.pressure_iters;
   if (N <= 0 || c->prm.solver == 1 || (c->prm.use_multigrid && c->prm.multigrid_levels > 1)) return FL_OK;
+  if (hipSetDevice(c->dev) != hipSuccess) return berr(FL_ERR_HIP, "hipSetDevice");
   int rc = flush_edits(c);
   if (rc) return rc;
   if (c->flags_dirty && (rc = launch_flags(c))) return rc;
-  int best = 0;
+  TuneShape best{};
   float best_ms = 1e30f;
-  const int saved_solver = c->prm.solver;
+  const fl_params saved = c->prm;
   c->prm.solver = 2;
-  for (int tk : {8, 16}) {
-    c->prm.block_sweeps = tk;
+  for (const TuneShape& sh : tune_shapes(tiles_at(c->NX, N, 8), device_cus(c->dev))) {
+    c->prm.block_sweeps = sh.tk;
+    c->prm.block_cols = sh.blk;
+    c->prm.handoff_lag = sh.lag;
     // the solve swaps U/U2, V/V2, p/T3 towards its outputs; swapping back restores the state
     float t = time_ms(c->stream,
                       [&]() {
@@ -507,15 +545,18 @@ int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms) {
                         return r;
                       },
                       2, &rc);
-    if (rc) return rc;
+    if (rc) break;
     if (t < best_ms) {
       best_ms = t;
-      best = tk;
+      best = sh;
     }
   }
-  c->prm.solver = saved_solver;
-  c->prm.block_sweeps = best;
-  if (block_sweeps) *block_sweeps = best;
+  c->prm = saved;
+  if (rc) return rc;
+  c->prm.block_sweeps = best.tk;
+  c->prm.block_cols = best.blk;
+  c->prm.handoff_lag = best.lag;
+  if (block_sweeps) *block_sweeps = best.tk;
   if (ms) *ms = best_ms;
   return FL_OK;
 }

@@ -377,6 +377,7 @@ fl_params default_params() {
   p.block_sweeps = 0;
   p.precise_advection = 0;
   p.handoff_lag = 0;
+  p.block_cols = 0;
   return p;
 }
 

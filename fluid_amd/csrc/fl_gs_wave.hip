@@ -39,7 +39,6 @@
 namespace fl {
 
 static constexpr int LANES = 64;
-static constexpr int BLK = 8;                  // columns per staging/publish block
 static constexpr int SPIN_LIMIT = 1 << 22;     // bounded wait (about a second); then error and continue
 
 // One launch of the solve (main run or early-exit re-run).
@@ -53,7 +52,8 @@ struct GsRun {
   const float* omega;
   float damping;
   int rPin, rPout;    // roles of the initial p (read; < 0: p starts at zero) and of the solve's p
-  int defer;          // publish handoff rows one block late (publish_rows below)
+  int defer;          // when handoff rows are published (publish_rows below): 0 at the next
+                      // block's start, 1 one block later, 2 halfway through the next block
 };
 
 __device__ __forceinline__ bool tile_empty(int g, int kb, int K, int TK, int NX) {
@@ -85,24 +85,43 @@ template <int N> __device__ __forceinline__ void wait_stores_but() {
   asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
 }
 
-__device__ __forceinline__ int wait_progress(int32_t* p, int need, int32_t* err) {
-  int v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int it = 0; v < need; it++) {
-    if (it > SPIN_LIMIT) {
+__device__ __forceinline__ int ld_prog(const int32_t* p) {
+  return __hip_atomic_load(const_cast<int32_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until producer progress reaches what a block needs: seen[x] >= need[x] for the G and K
+// producers (need 0: nothing to wait for). The counters still short are loaded together, so a
+// block whose producers have both published costs one round trip. The wait is bounded: after
+// SPIN_LIMIT polls, or as soon as another wait of this simulation has expired (err set), it
+// sets err and gives up, so a solve whose producers stall ends within about a second with
+// FL_ERR_INTERNAL instead of every later wait running out its own limit.
+__device__ __forceinline__ void wait_producers(const int32_t* pg, int needG, const int32_t* pk, int needK, int& seenG,
+                                               int& seenK, int32_t* err) {
+  for (int it = 0;; it++) {
+    const bool wG = seenG < needG, wK = seenK < needK;
+    if (!wG && !wK) return;
+    if (it > SPIN_LIMIT || ((it & 255) == 255 && ld_prog(err) != 0)) {
       atomicOr(err, 1);
-      return need;
+      seenG = max(seenG, needG);
+      seenK = max(seenK, needK);
+      return;
     }
-    __builtin_amdgcn_s_sleep(2);
-    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (it) __builtin_amdgcn_s_sleep(2);
+    int vG = 0, vK = 0;
+    if (wG) vG = ld_prog(pg);
+    if (wK) vK = ld_prog(pk);
+    if (wG) seenG = vG;
+    if (wK) seenK = vK;
   }
-  return v;
 }
 
 __device__ __forceinline__ const SimDesc& sim_desc(const OneSim& m, int) { return m.d; }
 __device__ __forceinline__ const SimDesc& sim_desc(const ManySims& m, int e) { return m.sims[e]; }
 
-template <int TK, class M>
+// TK sweeps per tile (one wave each), BLK columns per staging and publish block (BLK divides 64).
+template <int TK, int BLK, class M>
 __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
+  static_assert(LANES % BLK == 0, "progress is published in whole blocks that divide the tile height");
   const int lane = threadIdx.x & (LANES - 1);
   // wave-uniform, and held in a scalar register so every test on it is a scalar branch
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / LANES);
@@ -115,7 +134,6 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
   __shared__ float4 Gst[2][BLK][TK];
   __shared__ float4 Gos[2][BLK][TK];
   __shared__ int s_tile;
-  __shared__ int s_seen[3];
   // a re-run launch with nothing to re-run ends here, in every workgroup alike
   if (R.rerun && *R.rerun == 0) return;
 
@@ -157,36 +175,38 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     const int i0 = LANES * g - k0;           // row of (lane 0, wave 0)
     const int row = i0 + lane - wave;        // this thread's row
     const bool real_row = row >= 0 && row <= NX - 1;
-    const float om = wave_on ? R.omega[k] : 0.0f;
+    // this sweep's relaxation, read once into a scalar register: a vector-loaded value used in
+    // the step loop would make every step wait for all of the wave's memory operations
+    const float om = __int_as_float(
+        __builtin_amdgcn_readfirstlane(__float_as_int(wave_on ? R.omega[min(k, K - 1)] : 0.0f)));
     const int d0 = LANES * g - k0 - 2 * wave;  // anti-diagonal of column 0 for this wave
 
     // Producers: G stream (g-1,kb), K stream (g,kb-1), corner (g-1,kb-1).
     const int pG = (g > 0 && !tile_empty(g - 1, kb, K, TK, NX)) ? (g - 1) * KB + kb : -1;
     const int pK = (kb > 0 && !tile_empty(g, kb - 1, K, TK, NX)) ? g * KB + kb - 1 : -1;
     const int pC = (g > 0 && kb > 0 && !tile_empty(g - 1, kb - 1, K, TK, NX)) ? (g - 1) * KB + kb - 1 : -1;
-    if (tid < 3) s_seen[tid] = 0;
-    __syncthreads();
+    int seenG = 0, seenK = 0;  // producer progress last observed (thread 0)
 
-    // Wait until every producer has published the rows block `blk` stages.
+    // Wait until every producer has published the rows block `blk` stages: G stream rows up to
+    // s1 + 63, K stream rows up to s1 + TK - 1, corner rows up to s1 + 63 + TK (s1 = the block's
+    // last step). The corner producer (g-1, kb-1) is the G producer's K producer, and
+    // progress is published in whole blocks: G progress >= s1 + 64 = BLK (blk+1) + 63 means
+    // progress >= BLK (blk+1) + 64 = s1 + 65, which the G producer could publish only after it
+    // waited for corner progress >= s1 + 64 + TK (both clamped to rows). So with a G producer
+    // the corner needs no poll of its own; without one (g-1, kb) is empty, and the corner is
+    // polled in the G slot.
+    const int pGp = pG >= 0 ? pG : pC;
+    const int lagG = pG >= 0 ? LANES : LANES + TK;
     auto wait_block = [&](int blk) {
       if (tid == 0) {
         const int s1 = blk * BLK + BLK - 1;  // last step of the block
-        if (pG >= 0) {
-          int need = min(s1 + 63 + 1, rows);
-          if (s_seen[0] < need) s_seen[0] = wait_progress(&prog[pG], need, err);
-        }
-        if (pK >= 0) {
-          int need = min(s1 + TK - 1 + 1, rows);
-          if (s_seen[1] < need) s_seen[1] = wait_progress(&prog[pK], need, err);
-        }
-        if (pC >= 0) {
-          int need = min(s1 + 63 + TK + 1, rows);
-          if (s_seen[2] < need) s_seen[2] = wait_progress(&prog[pC], need, err);
-        }
+        const int needG = pGp >= 0 ? min(s1 + lagG, rows) : 0;
+        const int needK = pK >= 0 ? min(s1 + TK, rows) : 0;
+        wait_producers(prog + max(pGp, 0), needG, prog + max(pK, 0), needK, seenG, seenK, err);
       }
     };
 
-    // Staged boundary items of one block: K stream 8x65, G stream 8xTK.
+    // Staged boundary items of one block: K stream BLK x 65, G stream BLK x TK.
     // Items [0, KITEM) are the K stream; the G stream starts at the next wave boundary, so no
     // wave holds lanes of both (a wave running both branches would wait, in the second, for
     // the loads the first issued into the registers both use).
@@ -259,7 +279,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         }
       }
     };
-    // Flags of this thread's cells for the 8 columns of a block (anti-diagonal layout).
+    // Flags of this thread's cells for the BLK columns of a block (anti-diagonal layout).
     auto load_flags = [&](int blk, uint32_t* fl) {
 #pragma unroll
       for (int u = 0; u < BLK; u++) {
@@ -273,16 +293,19 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     // Producer side, in two halves run by the last wave (which also stores the K-stream
     // rows, three stores per step). publish_data(blk) copies the G stream of block blk to
     // global without waiting. publish_rows(n) makes handoff rows [0, n) visible: it waits
-    // for the wave's stores of those rows, then stores the progress counter. On time, the
-    // rows of block blk are published at the start of block blk + 1 after a full wait. With
-    // R.defer they are published at the start of block blk + 2, right after
+    // for the wave's stores of those rows, then stores the progress counter. On time
+    // (R.defer 0), the rows of block blk are published at the start of block blk + 1 after a
+    // full wait. With R.defer 1 they are published at the start of block blk + 2, right after
     // publish_data(blk + 1): by then the wave has issued at least NEWER stores after them
     // (the 3 * BLK K-stream stores of block blk + 1 and the G copy of block blk + 1), and
     // stores retire in issue order, so waiting until at most NEWER are in flight completes
-    // them without waiting for the stores just issued. Deferring removes a store round trip
-    // from every block and lengthens the pipeline fill by a block per producer hop. After
-    // the loop the last blocks are published with a full wait.
+    // them without waiting for the stores just issued. With R.defer 2 they are published
+    // after step MID - 1 of block blk + 1, when the 3 * MID K-stream stores of that block's
+    // first MID steps are the newer ones. Deferring removes a store round trip from every
+    // block and lengthens the pipeline fill by the steps of delay per producer hop. After the
+    // loop the last blocks are published with a full wait.
     constexpr int NEWER = 3 * BLK + 3 * ((BLK * TK + LANES - 1) / LANES);
+    constexpr int MID = BLK / 2;
     auto publish_data = [&](int blk) {
       if (wave == TK - 1) {
         for (int e = lane; e < BLK * TK; e += LANES) {
@@ -304,6 +327,12 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         if (lane == 0) __hip_atomic_store(&prog[me], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     };
+    auto publish_rows_mid = [&](int n) {
+      if (wave == TK - 1) {
+        wait_stores_but<3 * MID>();
+        if (lane == 0) __hip_atomic_store(&prog[me], n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    };
 
     // Prologue: block 0 staged synchronously.
     wait_block(0);
@@ -319,9 +348,9 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
       const int cb = blk & 1;
       if (blk > 0) {
         publish_data(blk - 1);
-        if (!R.defer)
+        if (R.defer == 0)
           publish_rows(blk * BLK, true);
-        else if (blk > 1)
+        else if (R.defer == 1 && blk > 1)
           publish_rows((blk - 1) * BLK, false);
       }
       const bool more = blk + 1 < nblk;
@@ -415,6 +444,9 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
           st_dev(kr + (LANES + 1) + lane + 1, Vb);
           st_dev(kr + 2 * (LANES + 1) + lane + 1, p);
         }
+        // halfway publication of the previous block's rows (R.defer 2): the K-stream stores of
+        // steps 0..MID-1 of this block are the 3 * MID newest of this wave
+        if (u == MID - 1 && R.defer == 2 && blk > 0) publish_rows_mid(blk * BLK);
         if (last) {  // only the wave of the final sweep writes the results
           const int j = s - lane - wave;
           if (real_row && j >= 0 && j <= NY - 1) {
@@ -526,16 +558,17 @@ static bool tile_empty_h(int g, int kb, int K, int TK, int NX) {
   return rmax < 0 || rmin > NX - 1;
 }
 
-// Tile geometry of one simulation for N sweeps with TK sweeps per tile.
+// Tile geometry of one simulation for N sweeps with tk sweeps per tile and blk columns per
+// block.
 struct TileGeom {
   int G, KB, rows;
 };
-static TileGeom geom(int NX, int NY, int N, int tk) {
+static TileGeom geom(int NX, int NY, int N, int tk, int blk) {
   TileGeom t;
   t.G = (NX + N - 1 + LANES - 1) / LANES;
   t.KB = (N + tk - 1) / tk;
   const int steps = NY + LANES - 1 + tk - 1;
-  t.rows = (steps + BLK - 1) / BLK * BLK;
+  t.rows = (steps + blk - 1) / blk * blk;
   return t;
 }
 static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, int NX, int N, int tk,
@@ -548,41 +581,77 @@ static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, 
       }
 }
 
-// Sweeps per tile: the knob if set; otherwise 16 when the 16-deep tiling yields enough tiles
-// to keep every CU busy (wider blocks of sweeps halve the handoff traffic), else 8 (twice the
-// tiles, shorter pipeline fill). `tiles16` counts the tiles at depth 16.
-int pick_tk(const fl_params& P, int N, long long tiles16) {
-  if (P.block_sweeps == 8 || P.block_sweeps == 16) return P.block_sweeps;
-  return (N >= 64 && tiles16 >= 512) ? 16 : 8;
+// Tile shapes the kernel is instantiated for: sweeps per tile (waves per workgroup) and columns
+// per staging/publish block.
+static bool shape_ok(int tk, int blk) {
+  return (tk == 2 || tk == 4 || tk == 8 || tk == 16) && (blk == 4 || blk == 8 || blk == 16);
 }
-// Publish the handoff rows one block late (handoff_lag 2), on time (1), or (0) late when the
-// tiles fill the persistent grid at least twice: then the solve is throughput-bound and the
-// per-block store wait costs more than the longer pipeline fill.
+// Sweeps per tile: the knob if set; otherwise 16 when the 16-deep tiling yields enough tiles
+// to keep every CU busy (wider blocks of sweeps halve the handoff traffic), 4 for solves of at
+// most 8 sweeps (Simulate's own count: one wave per SIMD, 300x251 frame 0.466 -> 0.447 ms),
+// else 8 (twice the tiles of 16, shorter pipeline fill). `tiles16` counts the tiles at depth 16.
+int pick_tk(const fl_params& P, int N, long long tiles16) {
+  if (P.block_sweeps == 2 || P.block_sweeps == 4 || P.block_sweeps == 8 || P.block_sweeps == 16)
+    return P.block_sweeps;
+  if (N >= 64 && tiles16 >= 512) return 16;
+  return N <= 8 ? 4 : 8;
+}
+// Columns per block: the knob if set, else 8.
+int pick_blk(const fl_params& P) { return (P.block_cols == 4 || P.block_cols == 16) ? P.block_cols : 8; }
+// When handoff rows are published: on time (handoff_lag 1: GsRun.defer 0), one block late
+// (2: defer 1) or halfway through the next block (3: defer 2); automatic (0) picks one block
+// late when the tiles fill the persistent grid at least twice (the solve is throughput-bound
+// and the per-block store wait costs more than the longer pipeline fill), else on time.
 static int defer_handoff(const fl_params& P, int ntiles, int grid) {
-  if (P.handoff_lag == 1 || P.handoff_lag == 2) return P.handoff_lag == 2;
+  if (P.handoff_lag >= 1 && P.handoff_lag <= 3) return P.handoff_lag - 1;
   return ntiles >= 2 * grid ? 1 : 0;
 }
 static long long tiles_at16(int NX, int N) { return (long long)((NX + N - 1 + 63) / 64) * ((N + 15) / 16); }
 
-template <int TK> static int occupancy_grid(int dev, int ntiles, bool many) {
+// The kernel of one tile shape and launch map, and its persistent grid for `ntiles` tiles: as
+// many workgroups as fit on the device at once (and no more than there are tiles).
+template <class M> struct GsKernels {
+  using Fn = void (*)(M, GsRun);
+  static Fn get(int tk, int blk) {
+#define FL_GS_CASE(T, B) \
+  if (tk == T && blk == B) return k_gs_wave<T, B, M>;
+    FL_GS_CASE(2, 4) FL_GS_CASE(2, 8) FL_GS_CASE(2, 16)
+    FL_GS_CASE(4, 4) FL_GS_CASE(4, 8) FL_GS_CASE(4, 16)
+    FL_GS_CASE(8, 4) FL_GS_CASE(8, 8) FL_GS_CASE(8, 16)
+    FL_GS_CASE(16, 4) FL_GS_CASE(16, 8) FL_GS_CASE(16, 16)
+#undef FL_GS_CASE
+    return nullptr;
+  }
+};
+template <class M> static int occupancy_grid(int dev, int tk, int blk, int ntiles) {
   int per_cu = 0;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 1;
-  if (many)
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<TK, ManySims>, TK * LANES, 0);
-  else
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gs_wave<TK, OneSim>, TK * LANES, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, GsKernels<M>::get(tk, blk), tk * LANES, 0);
   return std::max(1, std::min(ntiles, std::max(1, per_cu) * prop.multiProcessorCount));
+}
+template <class M> static int launch_gs(int tk, int blk, int grid, hipStream_t stream, const M& m, const GsRun& R) {
+  auto fn = GsKernels<M>::get(tk, blk);
+  if (!fn) {
+    set_error("no Gauss-Seidel kernel for " + std::to_string(tk) + " sweeps x " + std::to_string(blk) + " columns");
+    return FL_ERR_INTERNAL;
+  }
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(tk * LANES), 0, stream, m, R);
+  FL_HIP(hipGetLastError());
+  return FL_OK;
 }
 
 static int layout(Ctx* c, int N) {
   GsWork& W = c->gs;
   const int tk = pick_tk(c->prm, N, tiles_at16(c->NX, N));
-  if (W.n_alloc == N && W.tk == tk) return FL_OK;
+  const int blk = pick_blk(c->prm);
+  if (W.n_alloc == N && W.tk == tk && W.blk == blk) return FL_OK;
+  if (!shape_ok(tk, blk)) return FL_ERR_INTERNAL;
   FL_HIP(hipStreamSynchronize(c->stream));  // earlier solves may still use the old layout
   pressure_free(c);
-  const TileGeom t = geom(c->NX, c->NY, N, tk);
+  const TileGeom t = geom(c->NX, c->NY, N, tk, blk);
   W.tk = tk;
+  W.blk = blk;
   W.n_alloc = N;
   W.G = t.G;
   W.KB = t.KB;
@@ -602,7 +671,7 @@ static int layout(Ctx* c, int N) {
   FL_HIP(hipMalloc(&W.d_prog, sizeof(int32_t) * 2 * nt));
   FL_HIP(hipMalloc(&W.d_ticket, sizeof(int32_t) * 4));  // main, re-run, re-run flag
   FL_HIP(hipMemcpy(W.d_tile, tiles.data(), sizeof(int2) * W.ntiles, hipMemcpyHostToDevice));
-  W.grid = tk == 16 ? occupancy_grid<16>(c->dev, W.ntiles, false) : occupancy_grid<8>(c->dev, W.ntiles, false);
+  W.grid = occupancy_grid<OneSim>(c->dev, tk, blk, W.ntiles);
   return FL_OK;
 }
 
@@ -637,11 +706,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
     R.which = which;
     R.ticket = W.d_ticket + which;
     R.rerun = which ? W.d_ticket + 2 : nullptr;
-    if (W.tk == 16)
-      hipLaunchKernelGGL((k_gs_wave<16, OneSim>), dim3(W.grid), dim3(16 * LANES), 0, c->stream, m, R);
-    else
-      hipLaunchKernelGGL((k_gs_wave<8, OneSim>), dim3(W.grid), dim3(8 * LANES), 0, c->stream, m, R);
-    FL_HIP(hipGetLastError());
+    if ((rc = launch_gs(W.tk, W.blk, W.grid, c->stream, m, R))) return rc;
     if (which == 0 && runs > 1) {
       hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(LANES), 0, c->stream, m, 1, N, W.d_ticket + 2);
       FL_HIP(hipGetLastError());
@@ -674,6 +739,8 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   long long t16 = 0;
   for (const SimDesc& d : descs) t16 += tiles_at16(d.g.NX, N);
   const int tk = pick_tk(B->prm, N, t16);
+  const int blk = pick_blk(B->prm);
+  if (!shape_ok(tk, blk)) return FL_ERR_INTERNAL;
   auto fill = [&]() {
     for (size_t e = 0; e < descs.size(); e++) {
       const SimDesc& f = B->gs_fields[e];
@@ -686,7 +753,7 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
       d.prog = f.prog;
     }
   };
-  if (B->gs_n == N && B->tk == tk && B->gs_fields.size() == descs.size()) {
+  if (B->gs_n == N && B->tk == tk && B->blk == blk && B->gs_fields.size() == descs.size()) {
     fill();
     return FL_OK;
   }
@@ -703,7 +770,7 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   size_t kelems = 0, gelems = 0, nprog = 0;
   std::vector<TileGeom> tg(descs.size());
   for (size_t e = 0; e < descs.size(); e++) {
-    tg[e] = geom(descs[e].g.NX, descs[e].g.NY, N, tk);
+    tg[e] = geom(descs[e].g.NX, descs[e].g.NY, N, tk, blk);
     append_tiles(order, (int)e, descs[e].g.NX, N, tk, tg[e]);
     const size_t nt = (size_t)tg[e].G * tg[e].KB;
     kelems += nt * tg[e].rows * 3 * (LANES + 1);
@@ -736,9 +803,10 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
     po += 2 * nt;
   }
   B->tk = tk;
+  B->blk = blk;
   B->gs_n = N;
   B->gs_nprog = (int)nprog;
-  B->grid = tk == 16 ? occupancy_grid<16>(B->dev, B->ntiles, true) : occupancy_grid<8>(B->dev, B->ntiles, true);
+  B->grid = occupancy_grid<ManySims>(B->dev, tk, blk, B->ntiles);
   fill();
   return FL_OK;
 }
@@ -765,11 +833,8 @@ int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
     R.which = which;
     R.ticket = B->ticket + which;
     R.rerun = which ? B->ticket + 2 : nullptr;
-    if (B->tk == 16)
-      hipLaunchKernelGGL((k_gs_wave<16, ManySims>), dim3(B->grid), dim3(16 * LANES), 0, B->stream, m, R);
-    else
-      hipLaunchKernelGGL((k_gs_wave<8, ManySims>), dim3(B->grid), dim3(8 * LANES), 0, B->stream, m, R);
-    FL_HIP(hipGetLastError());
+    int rc = launch_gs(B->tk, B->blk, B->grid, B->stream, m, R);
+    if (rc) return rc;
     if (which == 0 && runs > 1) {
       // one wave per member, four per workgroup
       hipLaunchKernelGGL(k_gs_decide<ManySims>, dim3((E + 3) / 4), dim3(4 * LANES), 0, B->stream, m, E, N,

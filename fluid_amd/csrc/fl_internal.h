@@ -98,6 +98,7 @@ struct ManySims {
 // Pressure workspace of the temporal-blocked Gauss-Seidel kernel.
 struct GsWork {
   int tk = 0;              // sweeps per tile (waves per workgroup)
+  int blk = 0;             // columns per staging/publish block
   int n_alloc = -1;        // sweep count the layout was built for
   int G = 0, KB = 0;       // tiles along skewed rows and along sweeps
   int rows = 0;            // handoff rows per tile (= steps per tile)
@@ -236,7 +237,7 @@ struct Batch {
   int ncell_items = 0, nring_items = 0;
   std::vector<SimDesc> descs;  // what d_sims holds
   // batched Gauss-Seidel layout
-  int tk = 0, gs_n = -1, gs_nprog = 0;
+  int tk = 0, blk = 0, gs_n = -1, gs_nprog = 0;
   int ntiles = 0, grid = 0;
   std::vector<SimDesc> gs_fields;  // per member: G, KB, rows and its handoff/progress slices
   int2* d_tiles = nullptr;
@@ -328,6 +329,7 @@ int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const i
                       int nseg);
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
 int pick_tk(const fl_params& P, int N, long long tiles16);
+int pick_blk(const fl_params& P);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
 int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p from rPin (< 0: zero), to rPout
 

@@ -87,6 +87,7 @@ def _init_knobs(o):
     # absolute-position arithmetic, bit for bit).
     o.PreciseAdvection = False
     o.HandoffLag = 0     # temporal-blocked solve: 0 auto, 1 on time, 2 one block late
+    o.BlockCols = 0      # temporal-blocked solve: columns per block, 0 auto (8), 4, 8 or 16
 
 
 def _params_of(o):
@@ -106,6 +107,7 @@ def _params_of(o):
     p.block_sweeps = int(o.BlockSweeps)
     p.precise_advection = int(bool(o.PreciseAdvection))
     p.handoff_lag = int(o.HandoffLag)
+    p.block_cols = int(o.BlockCols)
     return p
 
 
@@ -153,12 +155,14 @@ class Fluid:
         check(self._L.fl_set_params(self._ctx, ctypes.byref(p)))
 
     def autotune(self, dt):
-        """Time the candidate Gauss-Seidel tile depths on the current state; keep the fastest."""
+        """Time the candidate Gauss-Seidel tile shapes (sweeps per tile, columns per block,
+        handoff lag) on the current state; keep the fastest. Returns (sweeps per tile, ms)."""
         self._push()
         tk, ms = ctypes.c_int(), ctypes.c_float()
         check(self._L.fl_autotune(self._ctx, dt, ctypes.byref(tk), ctypes.byref(ms)))
-        if tk.value:
-            self.BlockSweeps = tk.value
+        p = FlParams()
+        check(self._L.fl_get_params(self._ctx, ctypes.byref(p)))
+        self.BlockSweeps, self.BlockCols, self.HandoffLag = p.block_sweeps, p.block_cols, p.handoff_lag
         return tk.value, ms.value
 
     # ---- whole-field access ---------------------------------------------------------
@@ -445,11 +449,13 @@ class FluidBatch:
         check(self._L.fl_batch_set_graph(self._b, int(bool(on))))
 
     def autotune(self, dt):
+        """fl_batch_autotune: the batched solve's tile shape, timed on the live state."""
         self._push()
         tk, ms = ctypes.c_int(), ctypes.c_float()
         check(self._L.fl_batch_autotune(self._b, dt, ctypes.byref(tk), ctypes.byref(ms)))
-        if tk.value:
-            self.BlockSweeps = tk.value
+        p = FlParams()
+        check(self._L.fl_batch_get_params(self._b, ctypes.byref(p)))
+        self.BlockSweeps, self.BlockCols, self.HandoffLag = p.block_sweeps, p.block_cols, p.handoff_lag
         return tk.value, ms.value
 
 

@@ -68,11 +68,14 @@ typedef struct fl_params {
   int32_t pressure_iters;    /* sweeps per Simulate, default 8 */
   int32_t early_exit;        /* 1: stop after the first sweep with maxDiv < 1e-5 */
   int32_t solver;            /* 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel */
-  int32_t block_sweeps;      /* sweeps per temporal block (0 = tuned default) */
+  int32_t block_sweeps;      /* sweeps per temporal block: 0 auto, 2, 4, 8 or 16 */
   int32_t precise_advection; /* 1: back-trace in cell units (no reference counterpart), default 0 */
   int32_t handoff_lag;       /* temporal-blocked solve: 0 auto, 1 publish each block's handoff rows
                                 at the next block's start, 2 one block later (fewer waits, longer
-                                pipeline fill; auto picks 2 from two rounds of tiles up) */
+                                pipeline fill; auto picks 2 from two rounds of tiles up), 3 halfway
+                                through the next block */
+  int32_t block_cols;        /* temporal-blocked solve: columns per staging and publish block,
+                                0 auto (8), 4, 8 or 16 */
 } fl_params;
 
 /* Statistics of the last Simulate (no reference counterpart). */

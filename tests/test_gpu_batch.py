@@ -95,6 +95,29 @@ def test_batch_matches_solo(knobs, graph):
     b.close()
 
 
+SHAPES = [(tk, blk) for tk in (2, 4, 8, 16) for blk in (4, 8, 16)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["tk%d_blk%d" % s for s in SHAPES])
+def test_batch_every_shape_matches_solo(shape):
+    """The merged persistent launch with each tile shape the batch autotuner can pick (with a
+    deferred handoff publication) leaves every member bit-identical to stepping it alone
+    with the default shape."""
+    knobs = dict(KNOB_SETS["bfecc_conf"], BlockSweeps=0)
+    b, solo = make_pair(SIZES, knobs)
+    b.BlockSweeps, b.BlockCols = shape
+    b.HandoffLag = 2 + (shape[0] + shape[1]) % 2  # a block late or halfway
+    for step in range(3):
+        b.update(C.DT)
+        for s in solo:
+            s.update(C.DT)
+        for e, (m, s) in enumerate(zip(b.members, solo)):
+            same(m, s, "shape %s step %d member %d %s" % (shape, step, e, SIZES[e]))
+    b.close()
+    for s in solo:
+        s.close()
+
+
 def test_batch_member_vs_oracle():
     """One ensemble member against the oracle directly (no solo Fluid in between)."""
     sizes = [(90, 50), (33, 77)]

@@ -23,31 +23,80 @@ def same(g, o, where, fields=FIELDS):
                                  (where, k, int(bad.sum()), i, j, a[i, j], b[i, j]))
 
 
-@pytest.mark.parametrize("tk", [8, 16])
+# every tile shape the solver and its autotuner can pick: sweeps per tile x columns per block
+SHAPES = [(tk, blk) for tk in (2, 4, 8, 16) for blk in (4, 8, 16)]
+SHAPE_IDS = ["tk%d_blk%d" % s for s in SHAPES]
+_ORACLE = {}
+
+
+def many_tiles_reference(early):
+    """Initial state and the oracle's state after each of 3 steps: 700x300, 100 sweeps (15 to
+    50 blocks of sweeps, so consumers of empty G producers poll their corner), BFECC."""
+    if early not in _ORACLE:
+        w, h, n = 700, 300, 100
+        rng = np.random.default_rng(21)
+        st = {}
+        S = (rng.random((w + 2, h + 2)) > 0.05).astype(np.float32)
+        S[0, :] = S[:, 0] = S[:, -1] = 0
+        st["S"] = S
+        for k in ("U", "V", "newU", "newV"):
+            st[k] = (0.5 * rng.standard_normal((w + 2, h + 2))).astype(np.float32)
+        st["M"] = rng.random((w + 2, h + 2)).astype(np.float32)
+        o = OracleFluid(1000.0, w, h, 0.01)
+        o.load_state(st)
+        o.PressureIters, o.EarlyExit, o.UseBFECC = n, early, True
+        outs = []
+        for _ in range(3):
+            o.Simulate(0.1)
+            outs.append(o.state())
+        _ORACLE[early] = (st, outs)
+    return _ORACLE[early]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
 @pytest.mark.parametrize("early", [False, True])
-@pytest.mark.parametrize("lag", [1, 2], ids=["on_time", "one_block_late"])
-def test_many_tiles_bit_exact(tk, early, lag):
-    w, h, n = 700, 300, 100
-    rng = np.random.default_rng(21)
-    st = {}
-    S = (rng.random((w + 2, h + 2)) > 0.05).astype(np.float32)
-    S[0, :] = S[:, 0] = S[:, -1] = 0
-    st["S"] = S
-    for k in ("U", "V", "newU", "newV"):
-        st[k] = (0.5 * rng.standard_normal((w + 2, h + 2))).astype(np.float32)
-    st["M"] = rng.random((w + 2, h + 2)).astype(np.float32)
-    g, o = Fluid(1000.0, w, h, 0.01), OracleFluid(1000.0, w, h, 0.01)
-    for f in (g, o):
-        f.load_state(st)
-        f.PressureIters = n
-        f.EarlyExit = early
-        f.UseBFECC = True
-    g.BlockSweeps = tk
+@pytest.mark.parametrize("lag", [1, 2, 3], ids=["on_time", "one_block_late", "halfway"])
+def test_many_tiles_bit_exact(shape, early, lag):
+    st, outs = many_tiles_reference(early)
+    g = Fluid(1000.0, 700, 300, 0.01)
+    g.load_state(st)
+    g.PressureIters, g.EarlyExit, g.UseBFECC = 100, early, True
+    g.BlockSweeps, g.BlockCols = shape
     g.HandoffLag = lag  # both ways of publishing the tile handoffs
     for step in range(3):
         g.Simulate(0.1)
+        for k in FIELDS:
+            a, b = g.get(k), outs[step][k]
+            bad = a.view(np.uint32) != b.view(np.uint32)
+            assert not bad.any(), "step %d: %s differs at %d cells" % (step, k, int(bad.sum()))
+    g.close()
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
+def test_small_grids_every_shape(shape):
+    """Grids of one tile, one strip and a few tiles (the critical-path-bound sizes the tuner
+    tries shallow tiles on), every handoff publication, bit-exact with the oracle."""
+    for (w, h, n) in ((1, 1, 6), (37, 23, 5), (150, 90, 24), (40, 300, 70)):
+        rng = np.random.default_rng(w * 1000 + h)
+        st = {"S": (rng.random((w + 2, h + 2)) > 0.1).astype(np.float32)}
+        for k in ("U", "V", "newU", "newV"):
+            st[k] = (0.5 * rng.standard_normal((w + 2, h + 2))).astype(np.float32)
+        o = OracleFluid(1000.0, w, h, 0.01)
+        o.load_state(st)
+        o.PressureIters, o.EarlyExit = n, False
         o.Simulate(0.1)
-        same(g, o, "step %d" % step)
+        want = o.state()
+        for lag in (1, 2, 3):
+            g = Fluid(1000.0, w, h, 0.01)
+            g.load_state(st)
+            g.PressureIters, g.EarlyExit = n, False
+            g.BlockSweeps, g.BlockCols = shape
+            g.HandoffLag = lag
+            g.Simulate(0.1)
+            for k in FIELDS:
+                a, b = g.get(k), want[k]
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (w, h, n, lag, k)
+            g.close()
 
 
 def test_karman_full_size():

@@ -61,6 +61,7 @@ _vp, _i, _f, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_
 _ip, _fp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "fl_last_error": (ctypes.c_char_p, []),
+    "fl_ctx_last_error": (_i, [_vp, ctypes.c_char_p, _sz]),
     "fl_device_count": (_i, [_ip]),
     "fl_create": (_i, [_f, _i, _i, _f, ctypes.POINTER(_vp)]),
     "fl_destroy": (_i, [_vp]),

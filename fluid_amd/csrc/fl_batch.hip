@@ -519,6 +519,7 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
 }
 
 int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return berr(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   const int N = c->prm.pressure_iters;

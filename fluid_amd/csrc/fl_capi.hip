@@ -23,7 +23,13 @@
 namespace fl {
 
 static thread_local std::string g_err;
-void set_error(const std::string& msg) { g_err = msg; }
+static thread_local Ctx* g_err_ctx = nullptr;  // context of the call in progress on this thread
+void set_error(const std::string& msg) {
+  g_err = msg;
+  if (g_err_ctx) g_err_ctx->last_err = msg;
+}
+CtxErrScope::CtxErrScope(Ctx* c) : prev(g_err_ctx) { g_err_ctx = c; }
+CtxErrScope::~CtxErrScope() { g_err_ctx = prev; }
 
 static int err(int code, const std::string& msg) {
   set_error(msg);
@@ -69,39 +75,57 @@ static size_t stage_min(const Ctx* c) { return c->batch ? size_t(16) << 20 : siz
 bool copy_is_direct(const Ctx* c, size_t bytes) { return bytes < stage_min(c); }
 // A chunk's host copy on four threads: one thread copies (and first-touches fresh pages of a
 // caller's buffer) at about 12 GB/s, below what the DMA delivers.
+// If a thread cannot be started (std::system_error) its part is copied on the calling thread:
+// no exception leaves the C ABI.
 static void host_copy(char* d, const char* s, size_t len) {
   constexpr int T = 4;
   const size_t part = (len / T + 63) / 64 * 64;
   std::thread th[T - 1];
   for (int t = 1; t < T; t++) {
     const size_t a = std::min(len, t * part), b = std::min(len, (t + 1) * part);
-    th[t - 1] = std::thread([=] { std::memcpy(d + a, s + a, b - a); });
+    try {
+      th[t - 1] = std::thread([=] { std::memcpy(d + a, s + a, b - a); });
+    } catch (...) {
+      std::memcpy(d + a, s + a, b - a);
+    }
   }
   std::memcpy(d, s, std::min(len, part));
-  for (auto& x : th) x.join();
+  for (auto& x : th)
+    if (x.joinable()) x.join();
 }
+// On a failed call inside a staged copy, wait for the DMAs already queued on the stream before
+// returning, so that none still reads or writes the pinned buffers the next copy refills.
+#define FL_HIP_STAGED(expr)                                                             \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::fl::set_error(std::string(#expr) + ": " + hipGetErrorString(e_));               \
+      (void)hipStreamSynchronize(c->stream);                                            \
+      return FL_ERR_HIP;                                                                \
+    }                                                                                   \
+  } while (0)
 int copy_to_host(Ctx* c, void* dst, const void* src, size_t bytes) {
   if (bytes < stage_min(c)) {
     FL_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     FL_HIP(hipStreamSynchronize(c->stream));
     return FL_OK;
   }
-  if (!c->stage) FL_HIP(hipHostMalloc((void**)&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
+  if (!c->stage) FL_HIP_STAGED(hipHostMalloc((void**)&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
   for (auto& e : c->stage_ev)
-    if (!e) FL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!e) FL_HIP_STAGED(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const char* s = static_cast<const char*>(src);
   char* d = static_cast<char*>(dst);
   const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
   auto len = [&](size_t k) { return std::min(STAGE_CHUNK, bytes - k * STAGE_CHUNK); };
-  FL_HIP(hipMemcpyAsync(c->stage, s, len(0), hipMemcpyDeviceToHost, c->stream));
-  FL_HIP(hipEventRecord(c->stage_ev[0], c->stream));
+  FL_HIP_STAGED(hipMemcpyAsync(c->stage, s, len(0), hipMemcpyDeviceToHost, c->stream));
+  FL_HIP_STAGED(hipEventRecord(c->stage_ev[0], c->stream));
   for (size_t k = 0; k < nch; k++) {
     if (k + 1 < nch) {  // its buffer was emptied by the host copy of chunk k-1
       char* b = c->stage + ((k + 1) & 1) * STAGE_CHUNK;
-      FL_HIP(hipMemcpyAsync(b, s + (k + 1) * STAGE_CHUNK, len(k + 1), hipMemcpyDeviceToHost, c->stream));
-      FL_HIP(hipEventRecord(c->stage_ev[(k + 1) & 1], c->stream));
+      FL_HIP_STAGED(hipMemcpyAsync(b, s + (k + 1) * STAGE_CHUNK, len(k + 1), hipMemcpyDeviceToHost, c->stream));
+      FL_HIP_STAGED(hipEventRecord(c->stage_ev[(k + 1) & 1], c->stream));
     }
-    FL_HIP(hipEventSynchronize(c->stage_ev[k & 1]));
+    FL_HIP_STAGED(hipEventSynchronize(c->stage_ev[k & 1]));
     host_copy(d + k * STAGE_CHUNK, c->stage + (k & 1) * STAGE_CHUNK, len(k));
   }
   return FL_OK;
@@ -115,21 +139,21 @@ int copy_to_device(Ctx* c, void* dst, const void* src, size_t bytes) {
     FL_HIP(hipStreamSynchronize(c->stream));
     return FL_OK;
   }
-  if (!c->stage) FL_HIP(hipHostMalloc((void**)&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
+  if (!c->stage) FL_HIP_STAGED(hipHostMalloc((void**)&c->stage, 2 * STAGE_CHUNK, hipHostMallocDefault));
   for (auto& e : c->stage_ev)
-    if (!e) FL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!e) FL_HIP_STAGED(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const char* s = static_cast<const char*>(src);
   char* d = static_cast<char*>(dst);
   const size_t nch = (bytes + STAGE_CHUNK - 1) / STAGE_CHUNK;
   for (size_t k = 0; k < nch; k++) {
     const size_t len = std::min(STAGE_CHUNK, bytes - k * STAGE_CHUNK);
     char* b = c->stage + (k & 1) * STAGE_CHUNK;
-    if (k >= 2) FL_HIP(hipEventSynchronize(c->stage_ev[k & 1]));  // the DMA of chunk k-2 left b
+    if (k >= 2) FL_HIP_STAGED(hipEventSynchronize(c->stage_ev[k & 1]));  // the DMA of chunk k-2 left b
     host_copy(b, s + k * STAGE_CHUNK, len);
-    FL_HIP(hipMemcpyAsync(d + k * STAGE_CHUNK, b, len, hipMemcpyHostToDevice, c->stream));
-    FL_HIP(hipEventRecord(c->stage_ev[k & 1], c->stream));
+    FL_HIP_STAGED(hipMemcpyAsync(d + k * STAGE_CHUNK, b, len, hipMemcpyHostToDevice, c->stream));
+    FL_HIP_STAGED(hipEventRecord(c->stage_ev[k & 1], c->stream));
   }
-  FL_HIP(hipStreamSynchronize(c->stream));
+  FL_HIP_STAGED(hipStreamSynchronize(c->stream));
   return FL_OK;
 }
 
@@ -462,6 +486,16 @@ extern "C" {
 
 const char* fl_last_error(void) { return g_err.c_str(); }
 
+int fl_ctx_last_error(fl_ctx* ctx, char* buf, size_t n) {
+  if (!ctx || (!buf && n)) return err(FL_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return FL_OK;
+  const std::string& e = C_OF(ctx)->last_err;
+  const size_t k = std::min(n - 1, e.size());
+  std::memcpy(buf, e.data(), k);
+  buf[k] = 0;
+  return FL_OK;
+}
+
 int fl_device_count(int* n) {
   int k = 0;
   hipError_t e = hipGetDeviceCount(&k);
@@ -526,6 +560,7 @@ int fl_create(float density, int width, int height, float h, fl_ctx** out) {
 }
 
 int fl_destroy(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return FL_OK;
   if (C_OF(ctx)->batch) return err(FL_ERR_INVALID_ARG, "batch members are destroyed with their batch");
   free_ctx(C_OF(ctx));
@@ -533,22 +568,26 @@ int fl_destroy(fl_ctx* ctx) {
 }
 
 int fl_dims(fl_ctx* ctx, int* nx, int* ny) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   if (nx) *nx = C_OF(ctx)->NX;
   if (ny) *ny = C_OF(ctx)->NY;
   return FL_OK;
 }
 int fl_get_h(fl_ctx* ctx, float* h) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !h) return err(FL_ERR_INVALID_ARG, "null argument");
   *h = C_OF(ctx)->h;
   return FL_OK;
 }
 int fl_get_params(fl_ctx* ctx, fl_params* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
   *out = C_OF(ctx)->prm;
   return FL_OK;
 }
 int fl_set_params(fl_ctx* ctx, const fl_params* in) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !in) return err(FL_ERR_INVALID_ARG, "null argument");
   if (in->pressure_iters < 0 || in->pressure_iters > 100000)
     return err(FL_ERR_INVALID_ARG, "pressure_iters out of range");
@@ -572,6 +611,7 @@ static float* field_ptr(Ctx* c, int f) {
 }
 
 int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   float* dst = field_ptr(c, field);
@@ -596,6 +636,7 @@ int fl_upload(fl_ctx* ctx, int field, const float* src, size_t n) {
 }
 
 int fl_fill_cells(fl_ctx* ctx, int field, int i0, int i1, int j0, int j1, float value) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   float* dst = field_ptr(c, field);
@@ -612,6 +653,7 @@ int fl_fill_cells(fl_ctx* ctx, int field, int i0, int i1, int j0, int j1, float 
 }
 
 int fl_download(fl_ctx* ctx, int field, float* dst, size_t n) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   float* src = field_ptr(c, field);
@@ -623,17 +665,20 @@ int fl_download(fl_ctx* ctx, int field, float* dst, size_t n) {
 }
 
 int fl_simulate(fl_ctx* ctx, float dt) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   return simulate(C_OF(ctx), dt);
 }
 
 int fl_sync(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   FL_HIP(hipStreamSynchronize(C_OF(ctx)->stream));
   return FL_OK;
 }
 
 int fl_get_stats(fl_ctx* ctx, fl_stats* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   if (!out) return err(FL_ERR_INVALID_ARG, "null output");
   Ctx* c = C_OF(ctx);
@@ -660,6 +705,7 @@ int fl_get_stats(fl_ctx* ctx, fl_stats* out) {
 }
 
 int fl_set_timing(fl_ctx* ctx, int on) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   if (on && !c->ev[0])
@@ -671,6 +717,7 @@ int fl_set_timing(fl_ctx* ctx, int on) {
 }
 
 int fl_get_timing(fl_ctx* ctx, fl_timing* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
   Ctx* c = C_OF(ctx);
   *out = fl_timing{};
@@ -683,6 +730,7 @@ int fl_get_timing(fl_ctx* ctx, fl_timing* out) {
 }
 
 int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   FL_HIP(hipStreamSynchronize(c->stream));
@@ -696,10 +744,12 @@ int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count) {
 }
 
 int fl_set_solid(fl_ctx* ctx, int i, int j, int solid) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   return set_solid(C_OF(ctx), i, j, solid != 0);
 }
 int fl_is_solid(fl_ctx* ctx, int i, int j, int* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !out) return err(FL_ERR_INVALID_ARG, "null argument");
   Ctx* c = C_OF(ctx);
   int rc = check_xy(c, i, j);
@@ -708,6 +758,7 @@ int fl_is_solid(fl_ctx* ctx, int i, int j, int* out) {
   return FL_OK;
 }
 int fl_set_velocity(fl_ctx* ctx, int i, int j, float u, float v) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   int rc = check_xy(c, i, j);
@@ -718,6 +769,7 @@ int fl_set_velocity(fl_ctx* ctx, int i, int j, float u, float v) {
   return FL_OK;
 }
 int fl_add_smoke(fl_ctx* ctx, int i, int j, float s) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   int rc = check_xy(c, i, j);
@@ -726,15 +778,18 @@ int fl_add_smoke(fl_ctx* ctx, int i, int j, float s) {
   return FL_OK;
 }
 int fl_reset(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   return reset(C_OF(ctx));
 }
 int fl_apply_force(fl_ctx* ctx, int i, int j, float fx, float fy) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   apply_force(C_OF(ctx), i, j, fx, fy);
   return FL_OK;
 }
 int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int radius) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   if (radius <= 0) {
@@ -754,6 +809,7 @@ int fl_apply_force_radius(fl_ctx* ctx, int cx, int cy, float fx, float fy, int r
   return FL_OK;
 }
 int fl_apply_force_gaussian(fl_ctx* ctx, int cx, int cy, float fx, float fy, float sigma) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   if (!(sigma > 0.0f) || sigma > 1e6f) return err(FL_ERR_INVALID_ARG, "sigma must be in (0, 1e6]");
   Ctx* c = C_OF(ctx);
@@ -772,12 +828,14 @@ int fl_apply_force_gaussian(fl_ctx* ctx, int cx, int cy, float fx, float fy, flo
   return FL_OK;
 }
 int fl_set_circular_obstacle(fl_ctx* ctx, int cx, int cy, int radius) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   set_circular_obstacle(C_OF(ctx), cx, cy, radius);
   return FL_OK;
 }
 
 int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   if (!out) return err(FL_ERR_INVALID_ARG, "null output");
   Ctx* c = C_OF(ctx);
@@ -795,6 +853,7 @@ int fl_get_adaptive_time_step(fl_ctx* ctx, float basedt, float* out) {
   return FL_OK;
 }
 int fl_max_divergence(fl_ctx* ctx, float* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   if (!out) return err(FL_ERR_INVALID_ARG, "null output");
   Ctx* c = C_OF(ctx);
@@ -804,6 +863,7 @@ int fl_max_divergence(fl_ctx* ctx, float* out) {
 }
 
 int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int n, float* us, float* vs) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   if (n < 0 || (n > 0 && (!xs || !ys || !us || !vs))) return err(FL_ERR_INVALID_ARG, "bad arguments");
   if (n == 0) return FL_OK;
@@ -830,6 +890,7 @@ int fl_sample_velocity_batch(fl_ctx* ctx, const float* xs, const float* ys, int 
   return FL_OK;
 }
 int fl_sample_velocity(fl_ctx* ctx, float x, float y, float* u, float* v) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   return fl_sample_velocity_batch(ctx, &x, &y, 1, u, v);
 }
 
@@ -846,11 +907,14 @@ static int view(fl_ctx* ctx, int kind, float* out, size_t n, float* mn, float* m
   if ((rc = field_range(c, c->T3, 1, mn, mx))) return rc;
   return copy_to_host(c, out, c->T3, 4 * n);
 }
-int fl_vorticity(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) { return view(ctx, 0, out, n, mn, mx); }
+int fl_vorticity(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx)); return view(ctx, 0, out, n, mn, mx); }
 int fl_velocity_magnitude(fl_ctx* ctx, float* out, size_t n, float* mn, float* mx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   return view(ctx, 1, out, n, mn, mx);
 }
 int fl_pass(fl_ctx* ctx, int pass, float dt, int iters) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   int rc = ctx_prepare(c);
@@ -881,6 +945,7 @@ int fl_pass(fl_ctx* ctx, int pass, float dt, int iters) {
 
 // ---- scene layer (package main) ------------------------------------------------------------
 int fl_scene_set_walls(fl_ctx* ctx, int top, int bottom, int left, int right) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   c->wallTop = top;
@@ -890,11 +955,13 @@ int fl_scene_set_walls(fl_ctx* ctx, int top, int bottom, int left, int right) {
   return FL_OK;
 }
 int fl_scene_apply_walls(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   apply_wall_settings(C_OF(ctx));
   return FL_OK;
 }
 int fl_scene_clear_walls(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   for (int i = 0; i < c->NX; i++)
@@ -903,6 +970,7 @@ int fl_scene_clear_walls(fl_ctx* ctx) {
   return FL_OK;
 }
 int fl_scene_preset(fl_ctx* ctx, int preset, float lid_u) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   if (preset < 0 || preset > 2) return err(FL_ERR_INVALID_ARG, "preset must be 0 (jet), 1 (cavity) or 2 (karman)");
@@ -930,6 +998,7 @@ int fl_scene_preset(fl_ctx* ctx, int preset, float lid_u) {
   return FL_OK;
 }
 int fl_scene_set_jet(fl_ctx* ctx, int on, int x, int j_lo, int j_hi, float u, float smoke) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   if (on) {
@@ -948,6 +1017,7 @@ int fl_scene_set_jet(fl_ctx* ctx, int on, int x, int j_lo, int j_hi, float u, fl
   return FL_OK;
 }
 int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   (sink ? c->sinks : c->sources).push_back(Source{i, j, u, v});
@@ -956,6 +1026,7 @@ int fl_scene_add_source(fl_ctx* ctx, int sink, int i, int j, float u, float v) {
   return FL_OK;
 }
 int fl_scene_clear_sources(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   c->sources.clear();
@@ -965,6 +1036,7 @@ int fl_scene_clear_sources(fl_ctx* ctx) {
   return FL_OK;
 }
 int fl_scene_apply_sources(fl_ctx* ctx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   auto inb = [&](int i, int j) { return i >= 0 && i < c->NX && j >= 0 && j < c->NY; };
@@ -978,6 +1050,7 @@ int fl_scene_apply_sources(fl_ctx* ctx) {
   return FL_OK;
 }
 int fl_scene_update(fl_ctx* ctx, float dt) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   CTX_CHECK(ctx);
   Ctx* c = C_OF(ctx);
   int rc = flush_edits(c);
@@ -988,6 +1061,7 @@ int fl_scene_update(fl_ctx* ctx, float dt) {
   return simulate(c, dt, true);
 }
 int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   int dx = std::abs(x1 - x0), dy = -std::abs(y1 - y0);
@@ -1009,6 +1083,7 @@ int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid) {
   return FL_OK;
 }
 int fl_scene_smoke_brush(fl_ctx* ctx, int ci, int cj, int r, float strength) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = C_OF(ctx);
   for (int i = ci - r; i <= ci + r; i++)

@@ -188,6 +188,9 @@ struct Ctx {
   double t_total = 0, t_pressure = 0, t_advect = 0, t_other = 0;
   int t_steps = 0;
 
+  // Text of the last failing call on this context (fl_ctx_last_error).
+  std::string last_err;
+
   // Stats of the last solve.
   int last_n = 0;
   bool last_rollback_checked = true;
@@ -268,6 +271,15 @@ struct Batch {
   fl_params graph_prm{};
   int graph_gen = 0, graph_key_gen = -1;  // bumped when a launch shape changes
   int steps = 0;
+};
+
+// Errors of a call that takes a context are also recorded in that context, so that a caller
+// whose next call may run on another OS thread (a cgo binding) still reads them: each such
+// entry point opens a CtxErrScope for the duration of the call.
+struct CtxErrScope {
+  Ctx* prev;
+  explicit CtxErrScope(Ctx* c);
+  ~CtxErrScope();
 };
 
 // ---- host runtime shared by single contexts and batches (fl_capi.hip) ------------------

@@ -799,6 +799,8 @@ __global__ void k_max_reduce(Grid g, const uint8_t* __restrict__ flags, const fl
   if (i < g.NX && j < g.NY) {
     const int n = g.NY, c = i * n + j;
     uint32_t f = flags[c];
+    // CF_FLUID is S != 0; the reference tests S > 0 (fluid.go:536). S holds only +0.0 and 1.0
+    // (fl_upload rejects every other bit pattern, SetSolid writes those two), so they agree.
     if ((f & CF_INTERIOR) && (f & CF_FLUID)) {
       float a;
       if (kind == 0)

@@ -343,6 +343,7 @@ using namespace fl;
 extern "C" {
 
 int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   float* f = nullptr;
@@ -363,6 +364,7 @@ int fl_field_minmax(fl_ctx* ctx, int field, float* mn, float* mx) {
 }
 
 int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx || !rgba) return rerr(FL_ERR_INVALID_ARG, "null argument");
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (view < 0 || view > 3) return rerr(FL_ERR_INVALID_ARG, "view must be 0 smoke, 1 pressure, 2 speed, 3 vorticity");
@@ -407,6 +409,7 @@ int fl_render(fl_ctx* ctx, int view, int overlay, uint8_t* rgba, size_t nbytes) 
 
 int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age, const float* max_age,
                         uint8_t* keep) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
   if (n < 0 || (n > 0 && (!x || !y || !age || !max_age || !keep))) return rerr(FL_ERR_INVALID_ARG, "bad arguments");
   if (n == 0) return FL_OK;
@@ -421,6 +424,7 @@ int fl_advect_particles(fl_ctx* ctx, float dt, int n, float* x, float* y, float*
 
 int fl_advect_particles_noalloc(fl_ctx* ctx, float dt, int n, float* x, float* y, float* age,
                                 const float* max_age, uint8_t* keep) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return rerr(FL_ERR_INVALID_ARG, "null context");
   if (n < 0 || (n > 0 && (!x || !y || !age || !max_age || !keep))) return rerr(FL_ERR_INVALID_ARG, "bad arguments");
   if (n > FL_PARTICLES_NOALLOC_MAX)

@@ -232,9 +232,15 @@ class Fluid:
         check(self._L.fl_pass(self._ctx, PASSES[name], dt, iters))
 
     # ---- walls.go -----------------------------------------------------------------------
+    def last_error(self):
+        """The text of the last failing call on this context (fl_ctx_last_error)."""
+        buf = ctypes.create_string_buffer(512)
+        self._L.fl_ctx_last_error(self._ctx, buf, len(buf))
+        return buf.value.decode()
+
     def _idx(self, rc):
         if rc in (2, 3):
-            raise IndexError(self._L.fl_last_error().decode())
+            raise IndexError(self.last_error())
         check(rc)
 
     def SetSolid(self, i, j, value):

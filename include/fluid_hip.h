@@ -87,6 +87,11 @@ typedef struct fl_stats {
 } fl_stats;
 
 const char* fl_last_error(void);
+/* The text of the last failing call on `ctx` (the reference's panic text for FL_ERR_X_RANGE /
+ * FL_ERR_Y_RANGE, walls.go:6-11), copied into buf (n bytes, NUL-terminated, truncated). No
+ * reference counterpart: fl_last_error is per OS thread, and a cgo caller's next call may run
+ * on another thread; this one can be read from any thread. */
+int fl_ctx_last_error(fl_ctx* ctx, char* buf, size_t n);
 int fl_device_count(int* n);
 
 /* New(density, width, height, h) — fluid.go:42-68. S starts all zero (all solid). */

@@ -443,3 +443,93 @@ def test_particles_match_oracle():
         ox, oy, oa, om = [np.ascontiguousarray(a[keep]) for a in (ox, oy, oa, om)]
         g.update(C.DT)
         o.update(C.DT)
+
+
+def test_context_error_text_survives_thread_switches():
+    """Two threads fail calls on two contexts at once; each context keeps the reference's
+    exact panic text of its own last failure (walls.go:6-11), readable from any thread, while
+    the per-thread fl_last_error of the reading thread is unrelated."""
+    import threading
+    a, b = Fluid(1.0, 8, 8, 1.0), Fluid(1.0, 8, 8, 1.0)
+    go = threading.Barrier(2)
+
+    def fail(f, i, j, reps):
+        go.wait()
+        for _ in range(reps):
+            assert f._L.fl_set_solid(f._ctx, i, j, 1) == (2 if i > 9 else 3)
+    ta = threading.Thread(target=fail, args=(a, 17, 1, 500))
+    tb = threading.Thread(target=fail, args=(b, 1, 23, 500))
+    ta.start()
+    tb.start()
+    ta.join()
+    tb.join()
+    assert a.last_error() == "invalid x-index: 17"
+    assert b.last_error() == "invalid y-index: 23"
+    with pytest.raises(IndexError, match="^invalid x-index: -1$"):
+        a.SetVelocity(-1, 2, 0.0, 0.0)
+    assert b.last_error() == "invalid y-index: 23"
+
+
+def test_solid_mask_upload_rejects_other_values():
+    """S must hold exactly +0.0 or 1.0 (every writer of the reference produces those), so the
+    device's S != 0 tests agree with the reference's S > 0 ones (fluid.go:536): negative,
+    -0.0, NaN and fractional values are refused and the mask is left as it was."""
+    g = Fluid(1.0, 6, 5, 1.0)
+    g.set("S", np.ones((8, 7), np.float32))
+    for bad in (-1.0, -0.0, np.nan, 0.5):
+        S = np.ones((8, 7), np.float32)
+        S[3, 3] = bad
+        with pytest.raises(Exception):
+            g.set("S", S)
+        assert (g.get("S") == 1.0).all()
+
+
+def test_staged_frame_outputs_match_oracle():
+    """The per-frame outputs above the 2 MB staging threshold, whose host copies go through
+    the pinned chunked buffers: the image of an 800x700 grid (2.2 MB), 2^19 tracer particles
+    (2 MB per array) and 2^19 SampleVelocity points (4 MB of positions), against the oracle."""
+    g = C.jet_scene(lambda *a: Fluid(*a), 800, 700, 8, jet_rows=40)
+    o = C.jet_scene(lambda *a: OracleFluid(*a, threads=8), 800, 700, 8, jet_rows=40)
+    for f in (g, o):
+        f.SetCircularObstacle(200, 350, 30)
+    for _ in range(3):
+        g.update(C.DT)
+        o.update(C.DT)
+    for view in (0, 3):
+        a, b = g.render(view), o.render(view)
+        assert np.array_equal(a, b), view
+    n = 1 << 19
+    rng = np.random.default_rng(12)
+    h = np.float32(C.H)
+    x = (rng.random(n) * 803 - 1).astype(np.float32) * h
+    y = (rng.random(n) * 703 - 1).astype(np.float32) * h
+    us, vs = g.SampleVelocityBatch(x, y)
+    for k in rng.integers(0, n, 400):
+        ou, ov = o.SampleVelocity(float(x[k]), float(y[k]))
+        assert us[k] == np.float32(ou) and vs[k] == np.float32(ov), k
+    age = (rng.random(n) * 1.4).astype(np.float32)
+    mx = np.full(n, 1.5, np.float32)
+    gx, gy, ga = x.copy(), y.copy(), age.copy()
+    ox, oy, oa = x.copy(), y.copy(), age.copy()
+    kg = g.advect_particles(np.float32(C.DT), gx, gy, ga, mx)
+    ko = o.advect_particles(np.float32(C.DT), ox, oy, oa, mx.copy())
+    assert np.array_equal(kg, ko)
+    for a, b in ((gx, ox), (gy, oy), (ga, oa)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_batch_member_staged_round_trip():
+    """A batch member's field of 16 MB and more goes through the batch threshold's staged copy
+    (2048x2048 member, 16.8 MB per field), bit for bit both ways, beside a small member."""
+    from fluid_amd import FluidBatch
+    b = FluidBatch(C.DENSITY, [(2048, 2048), (30, 20)], C.H)
+    big = b.members[0]
+    a = np.random.default_rng(4).standard_normal((big.NumX, big.NumY)).astype(np.float32)
+    big.set("M", a)
+    assert np.array_equal(big.get("M").view(np.uint32), a.view(np.uint32))
+    small = b.members[1]
+    s = np.random.default_rng(5).standard_normal((small.NumX, small.NumY)).astype(np.float32)
+    small.set("U", s)
+    assert np.array_equal(small.get("U").view(np.uint32), s.view(np.uint32))
+    assert np.array_equal(big.get("M").view(np.uint32), a.view(np.uint32))
+    b.close()

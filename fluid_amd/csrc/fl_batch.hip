@@ -34,7 +34,7 @@ static void batch_free(Batch* B) {
   for (Ctx* c : B->members) free_ctx(c);
   for (float* p : B->pool)
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)B->flags_pool, (void*)B->diag_pool, (void*)B->ctrl_pool, (void*)B->maxdiv_pool,
+  for (void* p : {(void*)B->flags_pool, (void*)B->diag_pool, (void*)B->keep_pool, (void*)B->ctrl_pool, (void*)B->maxdiv_pool,
                   (void*)B->d_sims, (void*)B->d_cells, (void*)B->d_rings, (void*)B->d_tiles, (void*)B->kout,
                   (void*)B->gout, (void*)B->prog, (void*)B->ticket, (void*)B->d_omega, (void*)B->d_src,
                   (void*)B->d_src_seg, (void*)B->d_src_sim, (void*)B->d_ekeys, (void*)B->d_evals,
@@ -344,13 +344,14 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
     batch_free(B);
     return berr(code, m);
   };
-  size_t ncells = 0, ndiag = 0;
+  size_t ncells = 0, ndiag = 0, nkeep = 0;
   for (int e = 0; e < n; e++) {
     if (widths[e] < 1 || heights[e] < 1) return fail(FL_ERR_INVALID_ARG, "member sizes must be >= 1");
     if ((int64_t)(widths[e] + 2) * (heights[e] + 2) > (int64_t)1 << 28)
       return fail(FL_ERR_INVALID_ARG, "member grid too large (more than 2^28 cells)");
     ncells += (size_t)(widths[e] + 2) * (heights[e] + 2);
     ndiag += (size_t)(widths[e] + heights[e] + 3) * (widths[e] + 2);
+    nkeep += (size_t)(widths[e] + 2) * ((heights[e] + 2 + 63) / 64);
   }
   if (ncells * NBUF > ((size_t)1 << 31) * 8) return fail(FL_ERR_INVALID_ARG, "batch too large");
   B->ncells = ncells;
@@ -361,6 +362,7 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
     (void)hipMemsetAsync(B->pool[k], 0, sizeof(float) * ncells, B->stream);
   }
   if (hipMalloc(&B->flags_pool, ncells) != hipSuccess || hipMalloc(&B->diag_pool, ndiag) != hipSuccess ||
+      hipMalloc(&B->keep_pool, sizeof(uint32_t) * nkeep) != hipSuccess ||
       hipMalloc(&B->ctrl_pool, sizeof(int32_t) * 4 * n) != hipSuccess ||
       hipMalloc(&B->maxdiv_pool, sizeof(uint32_t) * 2 * MEMBER_NMAX * n) != hipSuccess ||
       hipMalloc(&B->d_sims, sizeof(SimDesc) * n) != hipSuccess ||
@@ -371,7 +373,7 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
   (void)hipMemsetAsync(B->maxdiv_pool, 0, sizeof(uint32_t) * 2 * MEMBER_NMAX * n, B->stream);
   std::vector<CellItem> cells;
   std::vector<RingItem> rings;
-  size_t off = 0, doff = 0;
+  size_t off = 0, doff = 0, koff = 0;
   for (int e = 0; e < n; e++) {
     Ctx* c = new Ctx();
     c->batch = B;
@@ -391,6 +393,7 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
     for (int k = 0; k < NBUF; k++) *fields[k] = B->pool[k] + off;
     c->flags = B->flags_pool + off;
     c->flags_diag = B->diag_pool + doff;
+    c->keep = B->keep_pool + koff;
     c->d_ctrl = B->ctrl_pool + 4 * e;
     c->d_maxdiv = B->maxdiv_pool + (size_t)2 * MEMBER_NMAX * e;
     c->nmax = MEMBER_NMAX;
@@ -404,6 +407,7 @@ int fl_batch_create(int n, const int* widths, const int* heights, float density,
     for (int blk = 0; blk < (m + 255) / 256; blk++) rings.push_back(RingItem{e, blk});
     off += c->N;
     doff += (size_t)(c->NX + c->NY - 1) * c->NX;
+    koff += (size_t)c->NX * ((c->NY + 63) / 64);
   }
   {  // item k runs on XCD k % 8: give each XCD a contiguous run of the list
     std::vector<CellItem> dealt(cells.size());

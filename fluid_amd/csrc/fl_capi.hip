@@ -363,6 +363,7 @@ void free_ctx(Ctx* c) {
       if (*b) (void)hipFree(*b);
     if (c->flags) (void)hipFree(c->flags);
     if (c->flags_diag) (void)hipFree(c->flags_diag);
+    if (c->keep) (void)hipFree(c->keep);
     if (c->d_ctrl) (void)hipFree(c->d_ctrl);
     if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
   }
@@ -545,6 +546,8 @@ int fl_create(float density, int width, int height, float h, fl_ctx** out) {
     if ((e = hipMemsetAsync(*b, 0, sizeof(float) * c->N, c->stream)) != hipSuccess) return fail(e, "hipMemset");
   }
   if ((e = hipMalloc(&c->flags, c->N)) != hipSuccess) return fail(e, "hipMalloc flags");
+  if ((e = hipMalloc(&c->keep, sizeof(uint32_t) * c->NX * ((c->NY + 63) / 64))) != hipSuccess)
+    return fail(e, "hipMalloc keep");
   if ((e = hipMalloc(&c->flags_diag, (size_t)(c->NX + c->NY - 1) * c->NX)) != hipSuccess)
     return fail(e, "hipMalloc flags_diag");
   // entries off the grid stay zero for the context's lifetime

@@ -101,6 +101,33 @@ __device__ __forceinline__ void st_cell(float* __restrict__ base, int idx, float
 __device__ __forceinline__ uint32_t ld_flag(const uint8_t* __restrict__ base, int idx) {
   return *(base + (uint32_t)idx);
 }
+// Wave-uniform: some active lane of the wavefront has p.
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// Two and three consecutive elements of a row in one load (global_load_dwordx2 / dwordx3),
+// 4-byte aligned. A wavefront's gather or stencil load costs the L1 about the same whatever its
+// width, so loading a cell's neighbours along j together cuts the vector-memory work per cell.
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f3u __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ f2u ld_pair(const float* __restrict__ base, int idx) {
+  return *reinterpret_cast<const f2u*>(reinterpret_cast<const char*>(base) + (uint32_t)idx * 4u);
+}
+__device__ __forceinline__ f3u ld_trip(const float* __restrict__ base, int idx) {
+  return *reinterpret_cast<const f3u*>(reinterpret_cast<const char*>(base) + (uint32_t)idx * 4u);
+}
+
+// The corners (x0,y0), (x1,y0), (x1,y1), (x0,y1) of a bilinear sample, y1 = y0 or y0 + 1,
+// both <= NY-1: one pair per row, starting at ys = min(y0, NY-2) so that it holds both columns.
+struct Corners {
+  float a, b, c, d;
+};
+__device__ __forceinline__ Corners ld_corners(const float* __restrict__ f, int x0, int x1, int y0, int y1,
+                                              int NY) {
+  const int ys = min(y0, NY - 2);
+  const f2u p0 = ld_pair(f, x0 * NY + ys), p1 = ld_pair(f, x1 * NY + ys);
+  const bool a0 = y0 == ys, a1 = y1 == ys;
+  return Corners{a0 ? p0.x : p0.y, a0 ? p1.x : p1.y, a1 ? p1.x : p1.y, a1 ? p0.x : p0.y};
+}
 
 // int(math.Floor(float64(v))) with NaN/overflow mapped to 0 and flagged (the
 // reference would panic on the resulting slice index).
@@ -115,8 +142,17 @@ __device__ __forceinline__ int floor_index(float v, int* bad) {
 
 // sampleField / sampleFieldFrom (fluid.go:357-398, 1055-1091). (dx,dy) is (0,hs) for
 // U, (hs,0) for V and (hs,hs) for M. Evaluation order matches the reference exactly.
-__device__ __forceinline__ float sample_field(float x, float y, const float* __restrict__ f, float dx, float dy,
-                                              const Grid& g, int* bad) {
+// Corner source of a sample: the field in global memory.
+struct GlobalSrc {
+  const float* __restrict__ f;
+  __device__ __forceinline__ Corners corners(int x0, int x1, int y0, int y1, int NY, bool) const {
+    return ld_corners(f, x0, x1, y0, y1, NY);
+  }
+};
+
+template <class S>
+__device__ __forceinline__ float sample_field(float x, float y, const S& src, float dx, float dy, const Grid& g,
+                                              int* bad) {
   x = go_maxf(go_minf(x, g.xmax), g.h);
   y = go_maxf(go_minf(y, g.ymax), g.h);
   int x0 = min(floor_index((x - dx) * g.h1, bad), g.NX - 1);
@@ -125,6 +161,7 @@ __device__ __forceinline__ float sample_field(float x, float y, const float* __r
   int y0 = min(floor_index((y - dy) * g.h1, bad), g.NY - 1);
   float ty = ((y - dy) - (float)y0 * g.h) * g.h1;
   int y1 = min(y0 + 1, g.NY - 1);
+  const bool exact = x0 >= 0 && y0 >= 0;  // else x1 or y1 may equal x0 or y0 below the clamp
   if (x0 < 0 || y0 < 0) {
     *bad = 1;
     x0 = max(x0, 0);
@@ -134,10 +171,8 @@ __device__ __forceinline__ float sample_field(float x, float y, const float* __r
   }
   float sx = 1.0f - tx;
   float sy = 1.0f - ty;
-  const int n = g.NY;
-  float a = ld_cell(f, x0 * n + y0), b = ld_cell(f, x1 * n + y0), c = ld_cell(f, x1 * n + y1),
-        d = ld_cell(f, x0 * n + y1);
-  return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
+  const Corners k = src.corners(x0, x1, y0, y1, g.NY, exact);
+  return sx * sy * k.a + tx * sy * k.b + tx * ty * k.c + sx * ty * k.d;
 }
 
 // Back-trace in cell units (the precise_advection parameter; the reference always samples
@@ -189,8 +224,9 @@ __device__ __forceinline__ bool axis_cells(int base, float d, int N, bool half, 
 }
 // The bilinear sample of sampleField (fluid.go:389-397) at (i + dX, j + dY); hx/hy: the
 // axis carries a half-cell sample offset.
-__device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, const float* __restrict__ f, bool hx,
-                                              bool hy, const Grid& g, int* bad) {
+template <class S>
+__device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, const S& src, bool hx, bool hy,
+                                              const Grid& g, int* bad) {
   int x0, y0;
   float tx, ty;
   axis_cells(i, dX, g.NX, hx, x0, tx, bad);
@@ -198,10 +234,17 @@ __device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, 
   const int x1 = min(x0 + 1, g.NX - 1), y1 = min(y0 + 1, g.NY - 1);
   float sx = 1.0f - tx;
   float sy = 1.0f - ty;
-  const int n = g.NY;
-  float a = ld_cell(f, x0 * n + y0), b = ld_cell(f, x1 * n + y0), c = ld_cell(f, x1 * n + y1),
-        d = ld_cell(f, x0 * n + y1);
-  return sx * sy * a + tx * sy * b + tx * ty * c + sx * ty * d;
+  const Corners k = src.corners(x0, x1, y0, y1, g.NY, true);  // x1 = min(x0 + 1, NX - 1), y1 likewise
+  return sx * sy * k.a + tx * sy * k.b + tx * ty * k.c + sx * ty * k.d;
+}
+
+__device__ __forceinline__ float sample_field(float x, float y, const float* __restrict__ f, float dx, float dy,
+                                              const Grid& g, int* bad) {
+  return sample_field(x, y, GlobalSrc{f}, dx, dy, g, bad);
+}
+__device__ __forceinline__ float sample_cells(int i, int j, float dX, float dY, const float* __restrict__ f, bool hx,
+                                              bool hy, const Grid& g, int* bad) {
+  return sample_cells(i, j, dX, dY, GlobalSrc{f}, hx, hy, g, bad);
 }
 
 }  // namespace fl

@@ -56,6 +56,9 @@ struct SimDesc {
   float* buf[NBUF];
   uint8_t* flags;
   uint8_t* diag;
+  uint32_t* keep;     // [NX][ceil(NY/64)]: per row and 64-column block, bit 0 some interior velocity
+                      // face is not advected (advection keeps newU/newV), bit 1 some interior cell
+                      // is solid (keeps newM); written by k_flags with the flags
   int32_t* ctrl;      // [4]: [1] re-run sweeps, [2] invalid sample flag, [3] GS wait timeout
   uint32_t* maxdiv;   // [2][nmax]
   float* kout;        // GS K-stream handoff planes
@@ -131,6 +134,7 @@ struct Ctx {
   float *U2 = nullptr, *V2 = nullptr, *M2 = nullptr;
   float *T0 = nullptr, *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;  // BFECC / curl scratch
   uint8_t* flags = nullptr;      // cell flags, i-major
+  uint32_t* keep = nullptr;      // SimDesc::keep
   uint8_t* flags_diag = nullptr; // cell flags, anti-diagonal-major [d=i+j][i]
   bool flags_dirty = true;
 
@@ -205,6 +209,7 @@ struct Ctx {
     for (int k = 0; k < NBUF; k++) d.buf[k] = b[k];
     d.flags = flags;
     d.diag = flags_diag;
+    d.keep = keep;
     d.ctrl = d_ctrl;
     d.maxdiv = d_maxdiv;
     d.kout = gs.d_kout;
@@ -228,6 +233,7 @@ struct Batch {
   float* pool[NBUF] = {};
   uint8_t* flags_pool = nullptr;
   uint8_t* diag_pool = nullptr;
+  uint32_t* keep_pool = nullptr;
   int32_t* ctrl_pool = nullptr;
   uint32_t* maxdiv_pool = nullptr;
   size_t ncells = 0;

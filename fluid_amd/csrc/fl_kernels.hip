@@ -15,6 +15,11 @@
 // batch of uneven grids runs each pass as one launch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "fl_internal.h"
 
 namespace fl {
@@ -132,6 +137,10 @@ template <class M> __global__ void k_flags(M m) {
   if (interior && fluid && (f & (CF_X0 | CF_X1 | CF_Y0 | CF_Y1))) f |= CF_GS;
   d->flags[c] = (uint8_t)f;
   d->diag[(size_t)(i + j) * NX + i] = (uint8_t)f;
+  // the keep map: a wavefront is one row's 64-column block (BX = 64 lanes from j = 64*block)
+  const bool kv = interior && !(fluid && (f & CF_X0) && (f & CF_Y0)), ks = interior && !fluid;
+  const uint32_t kb = (wave_any(kv) ? 1u : 0u) | (wave_any(ks) ? 2u : 0u);
+  if ((j & (BX - 1)) == 0) d->keep[i * ((NY + BX - 1) / BX) + j / BX] = kb;
 }
 
 int launch_flags(Ctx* c) {
@@ -343,88 +352,315 @@ int launch_borders(Ctx* c) {
 }
 
 // ---- semi-Lagrangian advection (fluid.go:291-434) ----------------------------------------
-__device__ __forceinline__ float avg_u(const float* __restrict__ U, int c, int n) {
-  return (ld_cell(U, c - 1) + ld_cell(U, c) + ld_cell(U, c + n - 1) + ld_cell(U, c + n)) * 0.25f;
+// Gathering every neighbour and bilinear corner of every cell from global memory keeps the L1's
+// address and data units ~90% busy while HBM runs at 3-4 TB/s. So the advection passes stage the
+// windows of the fields they read in LDS with coalesced loads, a tile of TI x 64 cells plus a halo
+// of HI rows and HJ columns (1 for fields read only by the face averages and the 3x3 clamp), and
+// read stencils and corners from there. Corners outside the window (back-traces longer than the
+// halo, near the lid) come from global memory, for those lanes only. Values are the same whichever
+// memory they come from, so the result is bit-identical.
+// TI is 16, 8 or 4 for one simulation (adv_rows), the work-list's 4 rows in a batch. A persistent
+// variant that prefetched the next tile's windows into registers while computing the current one
+// ran 20-30% slower: its registers cut occupancy, and any global load of the compute waits for the
+// prefetch, since gfx9 counts outstanding loads in order.
+// Halo measured at 4096^2 (cavity, BFECC; the six launches of a step): (HI, HJ) = (2, 1) 550 us,
+// (2, 2) 552 us, (4, 2) 567 us. A wider halo serves a few more corners from LDS but stages more
+// rows per tile than it saves.
+constexpr int HI = 2, HJ = 1;
+
+// A window of one field in LDS: grid rows [r0, r0 + R), columns [c0, c0 + C), row pitch C. Slots
+// outside the grid hold copies of the nearest in-grid cells (the staging clamps its addresses), so
+// slot (x + 1, y + 1) holds the cell (min(x + 1, NX - 1), min(y + 1, NY - 1)) that a bilinear
+// sample pairs with (x, y).
+template <int R, int C> struct Win {
+  const float* s;
+  int r0, c0;
+  // (i, j) in the window: the local row is below 256, so its product is a 24-bit multiply
+  __device__ __forceinline__ float at(int i, int j) const {
+    return s[((unsigned)(i - r0) & 255u) * (unsigned)C + (unsigned)(j - c0)];
+  }
+};
+// A window's elements in registers: wave w of WY loads rows w, w + WY, ... over the tile's 64 columns (the
+// row offset is wave-uniform, so addresses cost one add per element), and the 2*HJ halo columns of
+// all R rows come in one more load per thread.
+template <int R, int C, int WY> struct Stage {
+  static constexpr int H = (C - BX) / 2, RW = (R + WY - 1) / WY, PAD = R * C;
+  static_assert(C - BX == 2 * H && 2 * H * R <= BX * WY, "halo columns in one pass");
+  float v[RW], vh;
+  __device__ __forceinline__ void load(const float* __restrict__ f, int r0, int c0, int NX, int NY) {
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+    const int gj = min(max(c0 + H + lane, 0), NY - 1);
+#pragma unroll
+    for (int m = 0; m < RW; m++) {
+      const int r = wy + WY * m;
+      const int gi = min(max(r0 + min(r, R - 1), 0), NX - 1);
+      v[m] = ld_cell(f, gi * NY + gj);
+    }
+    const int e = min(wy * BX + lane, 2 * H * R - 1), hr = e / (2 * H), side = e - hr * (2 * H);
+    const int hi = min(max(r0 + hr, 0), NX - 1), hj = min(max(c0 + (side < H ? side : BX + side), 0), NY - 1);
+    vh = ld_cell(f, hi * NY + hj);
+  }
+  __device__ __forceinline__ void store(float* s) const {
+    const int wy = __builtin_amdgcn_readfirstlane(threadIdx.y), lane = threadIdx.x;
+#pragma unroll
+    for (int m = 0; m < RW; m++) {
+      const int r = wy + WY * m;
+      if (r < R) s[r * C + H + lane] = v[m];
+    }
+    const int e = wy * BX + lane;
+    if (e < 2 * H * R) {
+      const int hr = e / (2 * H), side = e - hr * (2 * H);
+      s[hr * C + (side < H ? side : BX + side)] = vh;
+    }
+  }
+};
+// Corner source of a sample: the LDS window where the 2x2 block at (x0, y0) lies in it, else global
+// memory (only waves with such a lane issue the global loads). exact: x1, y1 are x0 + 1, y0 + 1 up
+// to the grid's clamp (false only for the clamped negative indices of an invalid sample).
+template <int R, int C> struct WinSrc {
+  Win<R, C> w;
+  const float* __restrict__ f;
+  __device__ __forceinline__ Corners corners(int x0, int x1, int y0, int y1, int NY, bool exact) const {
+    const unsigned a = (unsigned)(x0 - w.r0), cs = (unsigned)(y0 - w.c0);
+    const bool in = exact && a < (unsigned)(R - 1) && cs < (unsigned)(C - 1);
+    // every lane reads the window (lanes outside it at slot 0), so no branch guards the LDS reads
+    const float* p = w.s + (in ? (a & 255u) * (unsigned)C + cs : 0u);
+    Corners k{p[0], p[C], p[C + 1], p[1]};
+    if (wave_any(!in) && !in) k = ld_corners(f, x0, x1, y0, y1, NY);
+    return k;
+  }
+};
+
+// A tile: its simulation and its first row and column.
+struct TileRef {
+  const SimDesc* d;
+  int i0, j0;
+};
+template <int TI> __device__ __forceinline__ TileRef tile_ref(const OneSim& m, int t, int ntiles) {
+  const int gx = (m.d.g.NY + BX - 1) / BX;
+  const int s = xcd_tile(t, ntiles);  // each XCD takes a contiguous band of tiles
+  const int by = s / gx, bx = s - by * gx;
+  return TileRef{&m.d, by * TI, bx * BX};
 }
-__device__ __forceinline__ float avg_v(const float* __restrict__ V, int c, int n) {
-  return (ld_cell(V, c - n) + ld_cell(V, c) + ld_cell(V, c - n + 1) + ld_cell(V, c + 1)) * 0.25f;
+template <int TI> __device__ __forceinline__ TileRef tile_ref(const ManySims& m, int t, int) {
+  static_assert(TI == BY, "work-list items are BY rows");
+  const CellItem it = m.cells[t];
+  return TileRef{&m.sims[it.sim], it.by * BY, it.bx * BX};
 }
 
+// The thread's cells are rows i0 + threadIdx.y + WY*k, k < NR, of column j0 + threadIdx.x. Their
+// per-cell global inputs are prefetched with the windows (clamped addresses for cells outside the
+// grid, which compute nothing).
+template <int NR, int WY> __device__ __forceinline__ uint32_t row_flags(const TileRef& r) {
+  static_assert(NR <= 4, "flags of up to 4 rows, one byte each");
+  const int NX = r.d->g.NX, NY = r.d->g.NY, j = min(r.j0 + (int)threadIdx.x, NY - 1);
+  uint32_t fr = 0;
+#pragma unroll
+  for (int k = 0; k < NR; k++)
+    fr |= ld_flag(r.d->flags, min(r.i0 + (int)threadIdx.y + k * WY, NX - 1) * NY + j) << (8 * k);
+  return fr;
+}
+// The cell-centred velocity of the smoke passes, U[ci], U[ci+n], V[ci], V[ci+1], loaded with the
+// windows; ring cells (which do not advect) read around cell (1,1).
+template <int NR, int WY> struct CellVel {
+  float u0[NR], u1[NR], v0[NR], v1[NR];
+  __device__ __forceinline__ void load(const TileRef& r) {
+    const int NX = r.d->g.NX, NY = r.d->g.NY, j = r.j0 + (int)threadIdx.x;
+    const float* U = r.d->buf[B_U];
+    const float* V = r.d->buf[B_V];
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int i = r.i0 + (int)threadIdx.y + k * WY;
+      const bool ring = i <= 0 || i >= NX - 1 || j <= 0 || j >= NY - 1;
+      const int ci = ring ? NY + 1 : i * NY + j;
+      const f2u vv = ld_pair(V, ci);
+      u0[k] = ld_cell(U, ci);
+      u1[k] = ld_cell(U, ci + NY);
+      v0[k] = vv.x;
+      v1[k] = vv.y;
+    }
+  }
+};
+
+// The stale values a pass keeps (newU/newV of faces it does not advect, newM of solid cells), for
+// the rows whose keep-map entry has `bit`: the entry is a wave-uniform scalar load, so these loads
+// are issued with the windows and no wave makes a second round trip for them.
+template <int NR, int WY>
+__device__ __forceinline__ void kept_rows(const TileRef& r, uint32_t bit, const float* __restrict__ a,
+                                          const float* __restrict__ b, float (&va)[NR], float (&vb)[NR]) {
+  const int NX = r.d->g.NX, NY = r.d->g.NY, wy = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int nb = (NY + BX - 1) / BX, c0 = min(r.j0 + (int)threadIdx.x, NY - 1);
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    const int i = r.i0 + wy + k * WY;
+    va[k] = 0.0f;
+    vb[k] = 0.0f;
+    if (i < NX && (r.d->keep[i * nb + r.j0 / BX] & bit)) {
+      va[k] = ld_cell(a, i * NY + c0);
+      if (b) vb[k] = ld_cell(b, i * NY + c0);
+    }
+  }
+}
+
+// The tile loop shared by the advection passes: stage the tile's windows, then compute from LDS.
+// P supplies the LDS windows (Lds), the staging registers (Pre), the per-cell inputs loaded with
+// them (Rows) and the compute. One tile per workgroup: a workgroup's loads are all issued before its
+// first wait, and the 8 workgroups resident per CU overlap each other's staging and compute.
+template <class M, class P> __device__ __forceinline__ void run_tile(const M& m, const typename P::Args& a, int ntiles) {
+  __shared__ typename P::Lds lds;
+  const TileRef r = tile_ref<P::TI>(m, blockIdx.x, ntiles);
+  typename P::Rows w;
+  {
+    typename P::Pre pre;
+    P::prefetch(r, a, pre, w);
+    P::store(pre, lds);
+  }
+  __syncthreads();
+  P::compute(r, a, lds, w);
+}
+static inline int one_sim_tiles(int NX, int NY, int TI) { return ((NY + BX - 1) / BX) * ((NX + TI - 1) / TI); }
+// Rows per tile of one simulation: 16 (four waves of four rows) where that leaves >= 32 tiles per
+// CU, else 8 where that leaves >= 16, else 4: the tail of the last round of workgroups stays short
+// on small grids. Measured at 4096^2 (cavity, BFECC, six launches): 16 rows x 4 waves 554 us;
+// 32 x 8 585; 16 x 8 610; 8 x 4 589; 8 x 2 588.
+static int adv_rows(const Ctx* c) {
+  static int cus[64];
+  const int dev = c->dev & 63;
+  int n = __atomic_load_n(&cus[dev], __ATOMIC_RELAXED);
+  if (n <= 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, c->dev) != hipSuccess || n <= 0) n = 256;
+    __atomic_store_n(&cus[dev], n, __ATOMIC_RELAXED);
+  }
+  if (one_sim_tiles(c->NX, c->NY, 16) >= 32 * n) return 16;
+  if (one_sim_tiles(c->NX, c->NY, 8) >= 16 * n) return 8;
+  return 4;
+}
+#define FL_ADV_LAUNCH(K, c, a)                                                                                      \
+  do {                                                                                                              \
+    const int ti_ = adv_rows(c), nt_ = one_sim_tiles((c)->NX, (c)->NY, ti_);                                        \
+    const OneSim m_{(c)->desc()};                                                                                   \
+    if (ti_ == 16)                                                                                                  \
+      hipLaunchKernelGGL((K<OneSim, 16, 4>), dim3(nt_), dim3(BX, 4), 0, (c)->stream, m_, a, nt_);                   \
+    else if (ti_ == 8)                                                                                              \
+      hipLaunchKernelGGL((K<OneSim, 8, 4>), dim3(nt_), dim3(BX, 4), 0, (c)->stream, m_, a, nt_);                    \
+    else                                                                                                            \
+      hipLaunchKernelGGL((K<OneSim, 4, 4>), dim3(nt_), dim3(BX, 4), 0, (c)->stream, m_, a, nt_);                    \
+  } while (0)
+
 // advectVelocity: copyBorder(newU,U), the advected faces, stale newU elsewhere; then U=newU.
-// (Ut,Vt) is the current velocity (traced and sampled); the result goes to newU/newV and
-// to (Uo,Vo).
-// Every load is issued unconditionally, from addresses in range whatever the cell (neighbour
-// reads of cells outside a face's geometric range use a base cell inside the grid; samples of
-// faces that are not advected clamp like any other), so a wave waits for two rounds of loads
-// (the face neighbourhood, then both bilinear samples) instead of one round per branch.
-template <class M> __global__ void k_advect_velocity(M m, int rUt, int rVt, int rUo, int rVo, float dt, int write_new) {
-  const SimDesc* d;
-  int i, j;
-  if (!cell_of(m, d, i, j)) return;
-  const Grid g = d->g;
-  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
-  const float* Ut = d->buf[rUt];
-  const float* Vt = d->buf[rVt];
-  float* nU = d->buf[B_NU];
-  float* nV = d->buf[B_NV];
-  const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = ld_flag(d->flags, c);
-  // geometric ranges of the advected faces (u: 1 <= j <= NY-2, v: 1 <= i <= NX-2, both i, j >= 1)
-  const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
-  const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
-  const int cu = gu ? c : n + 1, cv = gv ? c : n + 1;  // (1,1): its neighbourhood is in range
-  const float uc = ld_cell(Ut, c), vc = ld_cell(Vt, c);
-  const float v_at_u = avg_v(Vt, cu, n), u_at_v = avg_u(Ut, cv, n);
-  const float su = ring ? uc : ld_cell(nU, c), sv = ring ? vc : ld_cell(nV, c);  // kept where not advected
-  int badu = 0, badv = 0;
-  float ru, rv;
-  if (g.precise) {
-    ru = sample_cells(i, j, ((-dt) * uc) * g.h1, ((-dt) * v_at_u) * g.h1, Ut, false, true, g, &badu);
-    rv = sample_cells(i, j, ((-dt) * u_at_v) * g.h1, ((-dt) * vc) * g.h1, Vt, true, false, g, &badv);
-  } else {
-    float xu = (float)i * g.h;
-    float yu = (float)j * g.h + g.ha;
-    xu = xu - dt * uc;
-    yu = yu - dt * v_at_u;
-    float xv = (float)i * g.h + g.ha;
-    float yv = (float)j * g.h;
-    xv = xv - dt * u_at_v;
-    yv = yv - dt * vc;
-    ru = sample_field(xu, yu, Ut, 0.0f, g.hs, g, &badu);
-    rv = sample_field(xv, yv, Vt, g.hs, 0.0f, g, &badv);
+// (Ut,Vt) is the current velocity (traced and sampled, staged with halo (HI, HJ)); the result goes to
+// newU/newV and to (Uo,Vo). Faces outside a component's geometric range read the window around
+// their own cell (values no output uses). The stale newU/newV are read only by waves holding a
+// face that is kept (ring or not advected), as the keep map says: in open flow no wave reads them.
+struct AdvVelArgs {
+  int rUt, rVt, rUo, rVo;
+  float dt;
+  int write_new;
+};
+template <int TI_, int WY> struct AdvVelPass {
+  static constexpr int TI = TI_, NR = TI / WY, R = TI + 2 * HI, C = BX + 2 * HJ;
+  using Args = AdvVelArgs;
+  struct Lds {
+    float U[Stage<R, C, WY>::PAD], V[Stage<R, C, WY>::PAD];
+  };
+  struct Pre {
+    Stage<R, C, WY> u, v;
+  };
+  struct Rows {
+    uint32_t f;          // the NR rows' flags, a byte each
+    float su[NR], sv[NR];  // stale newU/newV, loaded for rows whose keep bit 0 is set
+  };
+  static __device__ __forceinline__ void prefetch(const TileRef& r, const Args& a, Pre& p, Rows& w) {
+    const int NX = r.d->g.NX, NY = r.d->g.NY;
+    w.f = row_flags<NR, WY>(r);
+    p.u.load(r.d->buf[a.rUt], r.i0 - HI, r.j0 - HJ, NX, NY);
+    p.v.load(r.d->buf[a.rVt], r.i0 - HI, r.j0 - HJ, NX, NY);
+    kept_rows<NR, WY>(r, 1u, r.d->buf[B_NU], r.d->buf[B_NV], w.su, w.sv);
   }
-  const float ou = au ? ru : su, ov = av ? rv : sv;
-  if (write_new) {
-    st_cell(nU, c, ou);
-    st_cell(nV, c, ov);
+  static __device__ __forceinline__ void store(const Pre& p, Lds& l) {
+    p.u.store(l.U);
+    p.v.store(l.V);
   }
-  st_cell(d->buf[rUo], c, ou);
-  st_cell(d->buf[rVo], c, ov);
-  if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
+  static __device__ __forceinline__ void compute(const TileRef& r, const Args& a, const Lds& l, const Rows& w) {
+    const SimDesc* d = r.d;
+    const Grid g = d->g;
+    const int NX = g.NX, NY = g.NY, n = NY;
+    const float dt = a.dt;
+    const Win<R, C> wU{l.U, r.i0 - HI, r.j0 - HJ}, wV{l.V, r.i0 - HI, r.j0 - HJ};
+    const WinSrc<R, C> srcU{wU, d->buf[a.rUt]}, srcV{wV, d->buf[a.rVt]};
+    float* nU = d->buf[B_NU];
+    float* nV = d->buf[B_NV];
+    float* Uo = d->buf[a.rUo];
+    float* Vo = d->buf[a.rVo];
+    const int j = r.j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int i = r.i0 + threadIdx.y + k * WY;
+      if (i >= NX || j >= NY) continue;
+      const int c = i * n + j;
+      const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
+      const uint32_t f = (w.f >> (8 * k)) & 0xffu;
+      // geometric ranges of the advected faces (u: 1 <= j <= NY-2, v: 1 <= i <= NX-2, both i, j >= 1)
+      const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
+      const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
+      const float uc = wU.at(i, j), vc = wV.at(i, j);
+      // avgV(i,j) and avgU(i,j) (fluid.go:336-350)
+      const float v_at_u = (wV.at(i - 1, j) + wV.at(i, j) + wV.at(i - 1, j + 1) + wV.at(i, j + 1)) * 0.25f;
+      const float u_at_v = (wU.at(i, j - 1) + wU.at(i, j) + wU.at(i + 1, j - 1) + wU.at(i + 1, j)) * 0.25f;
+      // kept where not advected: the ring's copyBorder, else the stale newU/newV
+      const float su = ring ? uc : w.su[k], sv = ring ? vc : w.sv[k];
+      int badu = 0, badv = 0;
+      float ru, rv;
+      if (g.precise) {
+        ru = sample_cells(i, j, ((-dt) * uc) * g.h1, ((-dt) * v_at_u) * g.h1, srcU, false, true, g, &badu);
+        rv = sample_cells(i, j, ((-dt) * u_at_v) * g.h1, ((-dt) * vc) * g.h1, srcV, true, false, g, &badv);
+      } else {
+        float xu = (float)i * g.h;
+        float yu = (float)j * g.h + g.ha;
+        xu = xu - dt * uc;
+        yu = yu - dt * v_at_u;
+        float xv = (float)i * g.h + g.ha;
+        float yv = (float)j * g.h;
+        xv = xv - dt * u_at_v;
+        yv = yv - dt * vc;
+        ru = sample_field(xu, yu, srcU, 0.0f, g.hs, g, &badu);
+        rv = sample_field(xv, yv, srcV, g.hs, 0.0f, g, &badv);
+      }
+      const float ou = au ? ru : su, ov = av ? rv : sv;
+      if (a.write_new) {
+        st_cell(nU, c, ou);
+        st_cell(nV, c, ov);
+      }
+      st_cell(Uo, c, ou);
+      st_cell(Vo, c, ov);
+      if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
+    }
+  }
+};
+template <class M, int TI, int WY> __global__ void __launch_bounds__(BX* WY) k_advect_velocity(M m, AdvVelArgs a, int ntiles) {
+  run_tile<M, AdvVelPass<TI, WY>>(m, a, ntiles);
 }
 int launch_advect_velocity(Ctx* c, const float* Ut, const float* Vt, float* Uo, float* Vo, float dt, bool write_new) {
-  const int a = role_of(c, Ut), b = role_of(c, Vt), e = role_of(c, Uo), f = role_of(c, Vo);
-  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_advect_velocity<OneSim>, g, bl, 0, s, m, a, b, e, f, dt, (int)write_new);
-  });
+  const AdvVelArgs a{role_of(c, Ut), role_of(c, Vt), role_of(c, Uo), role_of(c, Vo), dt, (int)write_new};
+  FL_ADV_LAUNCH(k_advect_velocity, c, a);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
-// clampToNeighbors (fluid.go:1094-1120) for an interior cell (the 3x3 block is in range),
-// in two parts: the 3x3 range of src (taken before the back-trace, whose loads it does not
-// depend on) and the clamp of the corrected value to it. The comparisons are the reference's,
-// so a NaN neighbour is skipped and a NaN centre leaves the value unclamped.
+// clampToNeighbors (fluid.go:1094-1120) for an interior cell: the 3x3 range of src (taken
+// before the back-trace, whose loads it does not depend on) and the clamp of the corrected
+// value to it. The comparisons are the reference's, so a NaN neighbour is skipped and a NaN
+// centre leaves the value unclamped.
 struct NbRange {
   float lo, hi;
 };
-__device__ __forceinline__ NbRange neighbor_range(const float* __restrict__ src, int c, int n) {
-  NbRange r{ld_cell(src, c), ld_cell(src, c)};
+template <class W> __device__ __forceinline__ NbRange neighbor_range(const W& w, int i, int j) {
+  NbRange r{w.at(i, j), w.at(i, j)};
 #pragma unroll
   for (int di = -1; di <= 1; di++)
 #pragma unroll
     for (int dj = -1; dj <= 1; dj++) {
-      float v = ld_cell(src, c + di * n + dj);
+      const float v = w.at(i + di, j + dj);
       if (v < r.lo) r.lo = v;
       if (v > r.hi) r.hi = v;
     }
@@ -436,150 +672,258 @@ __device__ __forceinline__ float clamp_to(float val, NbRange r) {
   return val;
 }
 
-// advectVelocityBFECC passes 2 and 3 (fluid.go:935-987): backward trace with the
-// original velocity sampling the forward result, then correction and clamp. Loads are
-// issued unconditionally from in-range addresses, as in k_advect_velocity.
-template <class M>
-__global__ void k_bfecc_velocity_corr(M m, int rUo, int rVo, int rUf, int rVf, int rUc, int rVc, float dt) {
-  const SimDesc* d;
-  int i, j;
-  if (!cell_of(m, d, i, j)) return;
-  const Grid g = d->g;
-  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
-  const float* Uo = d->buf[rUo];
-  const float* Vo = d->buf[rVo];
-  const float* Uf = d->buf[rUf];
-  const float* Vf = d->buf[rVf];
-  const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = ld_flag(d->flags, c);
-  const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
-  const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
-  const int cu = gu ? c : n + 1, cv = gv ? c : n + 1, ci = ring ? n + 1 : c;
-  const float ou = ld_cell(Uo, c), ov = ld_cell(Vo, c);
-  const float v_at_u = avg_v(Vo, cu, n), u_at_v = avg_u(Uo, cv, n);
-  const float fu = ld_cell(Uf, c), fv = ld_cell(Vf, c);
-  const NbRange nu = neighbor_range(Uo, ci, n), nv = neighbor_range(Vo, ci, n);
-  int badu = 0, badv = 0;
-  float ru, rv;
-  if (g.precise) {
-    ru = sample_cells(i, j, (dt * ou) * g.h1, (dt * v_at_u) * g.h1, Uf, false, true, g, &badu);
-    rv = sample_cells(i, j, (dt * u_at_v) * g.h1, (dt * ov) * g.h1, Vf, true, false, g, &badv);
-  } else {
-    float xu = (float)i * g.h;
-    float yu = (float)j * g.h + g.ha;
-    xu = xu + dt * ou;
-    yu = yu + dt * v_at_u;
-    float xv = (float)i * g.h + g.ha;
-    float yv = (float)j * g.h;
-    xv = xv + dt * u_at_v;
-    yv = yv + dt * ov;
-    ru = sample_field(xu, yu, Uf, 0.0f, g.hs, g, &badu);
-    rv = sample_field(xv, yv, Vf, g.hs, 0.0f, g, &badv);
+// advectVelocityBFECC passes 2 and 3 (fluid.go:935-987): backward trace with the original
+// velocity (Uo,Vo; windows with halo 1 for the face averages and the 3x3 clamp ranges) sampling
+// the forward result (Uf,Vf; windows with halo (HI, HJ)), then correction and clamp. Ring cells keep
+// the original value; their traces still flag invalid samples where the reference's would fault.
+struct CorrVelArgs {
+  int rUo, rVo, rUf, rVf, rUc, rVc;
+  float dt;
+};
+template <int TI_, int WY> struct CorrVelPass {
+  static constexpr int TI = TI_, NR = TI / WY, R = TI + 2 * HI, C = BX + 2 * HJ, R1 = TI + 2, C1 = BX + 2;
+  using Args = CorrVelArgs;
+  struct Lds {
+    float Uf[Stage<R, C, WY>::PAD], Vf[Stage<R, C, WY>::PAD], Uo[Stage<R1, C1, WY>::PAD], Vo[Stage<R1, C1, WY>::PAD];
+  };
+  struct Pre {
+    Stage<R1, C1, WY> uo, vo;
+    Stage<R, C, WY> uf, vf;
+  };
+  struct Rows {
+    uint32_t f;  // the NR rows' flags, a byte each
+  };
+  static __device__ __forceinline__ void prefetch(const TileRef& r, const Args& a, Pre& p, Rows& w) {
+    const int NX = r.d->g.NX, NY = r.d->g.NY;
+    w.f = row_flags<NR, WY>(r);
+    p.uo.load(r.d->buf[a.rUo], r.i0 - 1, r.j0 - 1, NX, NY);
+    p.vo.load(r.d->buf[a.rVo], r.i0 - 1, r.j0 - 1, NX, NY);
+    p.uf.load(r.d->buf[a.rUf], r.i0 - HI, r.j0 - HJ, NX, NY);
+    p.vf.load(r.d->buf[a.rVf], r.i0 - HI, r.j0 - HJ, NX, NY);
   }
-  const float bu = au ? ru : (ring ? fu : 0.0f), bv = av ? rv : (ring ? fv : 0.0f);
-  // interior cells: corr = orig - (bwd - orig)/2 clamped to the 3x3 of orig; the ring keeps orig
-  const float kcu = clamp_to(ou - (bu - ou) * 0.5f, nu);
-  const float kcv = clamp_to(ov - (bv - ov) * 0.5f, nv);
-  st_cell(d->buf[rUc], c, ring ? ou : kcu);
-  st_cell(d->buf[rVc], c, ring ? ov : kcv);
-  if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
+  static __device__ __forceinline__ void store(const Pre& p, Lds& l) {
+    p.uo.store(l.Uo);
+    p.vo.store(l.Vo);
+    p.uf.store(l.Uf);
+    p.vf.store(l.Vf);
+  }
+  static __device__ __forceinline__ void compute(const TileRef& r, const Args& a, const Lds& l, const Rows& w) {
+    const SimDesc* d = r.d;
+    const Grid g = d->g;
+    const int NX = g.NX, NY = g.NY, n = NY;
+    const float dt = a.dt;
+    const Win<R1, C1> wU{l.Uo, r.i0 - 1, r.j0 - 1}, wV{l.Vo, r.i0 - 1, r.j0 - 1};
+    const WinSrc<R, C> srcU{Win<R, C>{l.Uf, r.i0 - HI, r.j0 - HJ}, d->buf[a.rUf]},
+        srcV{Win<R, C>{l.Vf, r.i0 - HI, r.j0 - HJ}, d->buf[a.rVf]};
+    float* Uc = d->buf[a.rUc];
+    float* Vc = d->buf[a.rVc];
+    const int j = r.j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int i = r.i0 + threadIdx.y + k * WY;
+      if (i >= NX || j >= NY) continue;
+      const int c = i * n + j;
+      const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
+      const uint32_t f = (w.f >> (8 * k)) & 0xffu;
+      const bool gu = i >= 1 && j >= 1 && j <= NY - 2, gv = i >= 1 && j >= 1 && i <= NX - 2;
+      const bool au = gu && (f & CF_FLUID) && (f & CF_X0), av = gv && (f & CF_FLUID) && (f & CF_Y0);
+      const float ou = wU.at(i, j), ov = wV.at(i, j);
+      const float v_at_u = (wV.at(i - 1, j) + wV.at(i, j) + wV.at(i - 1, j + 1) + wV.at(i, j + 1)) * 0.25f;
+      const float u_at_v = (wU.at(i, j - 1) + wU.at(i, j) + wU.at(i + 1, j - 1) + wU.at(i + 1, j)) * 0.25f;
+      const NbRange nu = neighbor_range(wU, i, j), nv = neighbor_range(wV, i, j);
+      int badu = 0, badv = 0;
+      float ru, rv;
+      if (g.precise) {
+        ru = sample_cells(i, j, (dt * ou) * g.h1, (dt * v_at_u) * g.h1, srcU, false, true, g, &badu);
+        rv = sample_cells(i, j, (dt * u_at_v) * g.h1, (dt * ov) * g.h1, srcV, true, false, g, &badv);
+      } else {
+        float xu = (float)i * g.h;
+        float yu = (float)j * g.h + g.ha;
+        xu = xu + dt * ou;
+        yu = yu + dt * v_at_u;
+        float xv = (float)i * g.h + g.ha;
+        float yv = (float)j * g.h;
+        xv = xv + dt * u_at_v;
+        yv = yv + dt * ov;
+        ru = sample_field(xu, yu, srcU, 0.0f, g.hs, g, &badu);
+        rv = sample_field(xv, yv, srcV, g.hs, 0.0f, g, &badv);
+      }
+      // interior cells: corr = orig - (bwd - orig)/2 clamped to the 3x3 of orig, bwd being 0 at
+      // faces the backward pass does not write; the ring keeps orig
+      const float bu = au ? ru : 0.0f, bv = av ? rv : 0.0f;
+      const float kcu = clamp_to(ou - (bu - ou) * 0.5f, nu);
+      const float kcv = clamp_to(ov - (bv - ov) * 0.5f, nv);
+      st_cell(Uc, c, ring ? ou : kcu);
+      st_cell(Vc, c, ring ? ov : kcv);
+      if ((au && badu) || (av && badv)) atomicOr(d->ctrl + 2, 1);
+    }
+  }
+};
+template <class M, int TI, int WY> __global__ void __launch_bounds__(BX* WY) k_bfecc_velocity_corr(M m, CorrVelArgs a, int ntiles) {
+  run_tile<M, CorrVelPass<TI, WY>>(m, a, ntiles);
 }
 int launch_bfecc_velocity_corr(Ctx* c, const float* Uorig, const float* Vorig, const float* Ufwd, const float* Vfwd,
                                float* Ucorr, float* Vcorr, float dt) {
-  const int a = role_of(c, Uorig), b = role_of(c, Vorig), e = role_of(c, Ufwd), f = role_of(c, Vfwd),
-            x = role_of(c, Ucorr), y = role_of(c, Vcorr);
-  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_bfecc_velocity_corr<OneSim>, g, bl, 0, s, m, a, b, e, f, x, y, dt);
-  });
+  const CorrVelArgs a{role_of(c, Uorig), role_of(c, Vorig), role_of(c, Ufwd), role_of(c, Vfwd),
+                      role_of(c, Ucorr), role_of(c, Vcorr), dt};
+  FL_ADV_LAUNCH(k_bfecc_velocity_corr, c, a);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
-// advectSmoke (fluid.go:400-434) using the current (already advected) velocity. Loads are
-// issued unconditionally from in-range addresses (ring cells read around cell (1,1)).
-template <class M>
-__global__ void k_advect_smoke(M m, int rMs, int rMo, float dt, float smokeAdv, float visc, int write_new) {
-  const SimDesc* d;
-  int i, j;
-  if (!cell_of(m, d, i, j)) return;
-  const Grid g = d->g;
-  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
-  const float* U = d->buf[B_U];
-  const float* V = d->buf[B_V];
-  const float* Ms = d->buf[rMs];
-  float* nM = d->buf[B_NM];
-  const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = ld_flag(d->flags, c);
-  const bool adv = (f & CF_INTERIOR) && (f & CF_FLUID);
-  const int ci = ring ? n + 1 : c;
-  const float mc = ld_cell(Ms, c);
-  const float keep = ring ? mc : ld_cell(nM, c);
-  float u = (ld_cell(U, ci) + ld_cell(U, ci + n)) * 0.5f * smokeAdv;
-  float v = (ld_cell(V, ci) + ld_cell(V, ci + 1)) * 0.5f * smokeAdv;
-  float nb = 0.0f;
-  if (visc > 0.0f)
-    nb = ld_cell(Ms, ci - n) + ld_cell(Ms, ci + n) + ld_cell(Ms, ci - 1) + ld_cell(Ms, ci + 1) - 4.0f * ld_cell(Ms, ci);
-  int badl = 0;
-  float x = (float)i * g.h + g.hs - dt * u;
-  float y = (float)j * g.h + g.hs - dt * v;
-  float val = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, Ms, true, true, g, &badl)
-                        : sample_field(x, y, Ms, g.hs, g.hs, g, &badl);
-  if (visc > 0.0f) {
-    float sd = visc * 0.3f * dt;
-    val += sd * nb;
+// advectSmoke (fluid.go:400-434) using the current (already advected) velocity (CellVel); the
+// source field is staged with halo (HI, HJ). The stale newM is loaded only for rows holding a solid
+// interior cell (keep map).
+struct AdvSmokeArgs {
+  int rMs, rMo;
+  float dt, smokeAdv, visc;
+  int write_new;
+};
+template <int TI_, int WY> struct AdvSmokePass {
+  static constexpr int TI = TI_, NR = TI / WY, R = TI + 2 * HI, C = BX + 2 * HJ;
+  using Args = AdvSmokeArgs;
+  struct Lds {
+    float M[Stage<R, C, WY>::PAD];
+  };
+  struct Pre {
+    Stage<R, C, WY> m;
+  };
+  struct Rows {
+    uint32_t f;  // the NR rows' flags, a byte each
+    CellVel<NR, WY> vel;
+    float mn[NR];  // stale newM, loaded for rows whose keep bit 1 is set
+  };
+  static __device__ __forceinline__ void prefetch(const TileRef& r, const Args& a, Pre& p, Rows& w) {
+    w.f = row_flags<NR, WY>(r);
+    w.vel.load(r);
+    p.m.load(r.d->buf[a.rMs], r.i0 - HI, r.j0 - HJ, r.d->g.NX, r.d->g.NY);
+    float unused[NR];
+    kept_rows<NR, WY>(r, 2u, r.d->buf[B_NM], nullptr, w.mn, unused);
   }
-  const float out = adv ? go_maxf(val, 0.0f) : keep;
-  if (write_new) st_cell(nM, c, out);
-  st_cell(d->buf[rMo], c, out);
-  if (adv && badl) atomicOr(d->ctrl + 2, 1);
+  static __device__ __forceinline__ void store(const Pre& p, Lds& l) { p.m.store(l.M); }
+  static __device__ __forceinline__ void compute(const TileRef& r, const Args& a, const Lds& l, const Rows& w) {
+    const SimDesc* d = r.d;
+    const Grid g = d->g;
+    const int NX = g.NX, NY = g.NY, n = NY;
+    const float dt = a.dt;
+    const Win<R, C> wM{l.M, r.i0 - HI, r.j0 - HJ};
+    const WinSrc<R, C> srcM{wM, d->buf[a.rMs]};
+    float* nM = d->buf[B_NM];
+    float* Mo = d->buf[a.rMo];
+    const int j = r.j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int i = r.i0 + threadIdx.y + k * WY;
+      if (i >= NX || j >= NY) continue;
+      const int c = i * n + j;
+      const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
+      const uint32_t f = (w.f >> (8 * k)) & 0xffu;
+      const bool adv = (f & CF_INTERIOR) && (f & CF_FLUID);
+      float u = (w.vel.u0[k] + w.vel.u1[k]) * 0.5f * a.smokeAdv;
+      float v = (w.vel.v0[k] + w.vel.v1[k]) * 0.5f * a.smokeAdv;
+      float nb = 0.0f;
+      if (a.visc > 0.0f)  // used at interior cells only
+        nb = wM.at(i - 1, j) + wM.at(i + 1, j) + wM.at(i, j - 1) + wM.at(i, j + 1) - 4.0f * wM.at(i, j);
+      const float keep = ring ? wM.at(i, j) : w.mn[k];
+      int badl = 0;
+      float x = (float)i * g.h + g.hs - dt * u;
+      float y = (float)j * g.h + g.hs - dt * v;
+      float val = g.precise ? sample_cells(i, j, ((-dt) * u) * g.h1, ((-dt) * v) * g.h1, srcM, true, true, g, &badl)
+                            : sample_field(x, y, srcM, g.hs, g.hs, g, &badl);
+      if (a.visc > 0.0f) {
+        float sd = a.visc * 0.3f * dt;
+        val += sd * nb;
+      }
+      const float out = adv ? go_maxf(val, 0.0f) : keep;
+      if (a.write_new) st_cell(nM, c, out);
+      st_cell(Mo, c, out);
+      if (adv && badl) atomicOr(d->ctrl + 2, 1);
+    }
+  }
+};
+template <class M, int TI, int WY> __global__ void __launch_bounds__(BX* WY) k_advect_smoke(M m, AdvSmokeArgs a, int ntiles) {
+  run_tile<M, AdvSmokePass<TI, WY>>(m, a, ntiles);
 }
 int launch_advect_smoke(Ctx* c, const float* Msrc, float* Mo, float dt, bool write_new) {
-  const int a = role_of(c, Msrc), b = role_of(c, Mo);
-  const float sa = c->prm.smoke_advection, visc = c->prm.viscosity_diffusion;
-  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_advect_smoke<OneSim>, g, bl, 0, s, m, a, b, dt, sa, visc, (int)write_new);
-  });
+  const AdvSmokeArgs a{role_of(c, Msrc), role_of(c, Mo), dt, c->prm.smoke_advection, c->prm.viscosity_diffusion,
+                       (int)write_new};
+  FL_ADV_LAUNCH(k_advect_smoke, c, a);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
-// advectSmokeBFECC passes 2 and 3 (fluid.go:1013-1046). Loads are issued unconditionally
-// from in-range addresses (ring cells read around cell (1,1)).
-template <class M> __global__ void k_bfecc_smoke_corr(M m, int rMo, int rMf, int rMc, float dt, float smokeAdv) {
-  const SimDesc* d;
-  int i, j;
-  if (!cell_of(m, d, i, j)) return;
-  const Grid g = d->g;
-  const int NX = g.NX, NY = g.NY, n = NY, c = i * n + j;
-  const float* U = d->buf[B_U];
-  const float* V = d->buf[B_V];
-  const float* Mo = d->buf[rMo];
-  const float* Mf = d->buf[rMf];
-  const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
-  const uint32_t f = ld_flag(d->flags, c);
-  const bool adv = !ring && (f & CF_FLUID);
-  const int ci = ring ? n + 1 : c;
-  const float om = ld_cell(Mo, c);
-  float u = (ld_cell(U, ci) + ld_cell(U, ci + n)) * 0.5f * smokeAdv;
-  float v = (ld_cell(V, ci) + ld_cell(V, ci + 1)) * 0.5f * smokeAdv;
-  const NbRange nr = neighbor_range(Mo, ci, n);
-  int badl = 0;
-  float x = (float)i * g.h + g.hs + dt * u;
-  float y = (float)j * g.h + g.hs + dt * v;
-  const float bs = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, Mf, true, true, g, &badl)
-                             : sample_field(x, y, Mf, g.hs, g.hs, g, &badl);
-  const float bm = adv ? bs : 0.0f;  // solid interior cells: the backward buffer's zero
-  float cm = clamp_to(om - (bm - om) * 0.5f, nr);
-  if (cm < 0.0f) cm = 0.0f;
-  st_cell(d->buf[rMc], c, ring ? om : cm);
-  if (adv && badl) atomicOr(d->ctrl + 2, 1);
+// advectSmokeBFECC passes 2 and 3 (fluid.go:1013-1046): the original smoke staged with halo 1
+// (centre and 3x3 clamp range), the forward result with halo (HI, HJ) (backward sample); the velocity
+// as in advectSmoke.
+struct CorrSmokeArgs {
+  int rMo, rMf, rMc;
+  float dt, smokeAdv;
+};
+template <int TI_, int WY> struct CorrSmokePass {
+  static constexpr int TI = TI_, NR = TI / WY, R = TI + 2 * HI, C = BX + 2 * HJ, R1 = TI + 2, C1 = BX + 2;
+  using Args = CorrSmokeArgs;
+  struct Lds {
+    float Mf[Stage<R, C, WY>::PAD], Mo[Stage<R1, C1, WY>::PAD];
+  };
+  struct Pre {
+    Stage<R1, C1, WY> mo;
+    Stage<R, C, WY> mf;
+  };
+  struct Rows {
+    uint32_t f;  // the NR rows' flags, a byte each
+    CellVel<NR, WY> vel;
+  };
+  static __device__ __forceinline__ void prefetch(const TileRef& r, const Args& a, Pre& p, Rows& w) {
+    const int NX = r.d->g.NX, NY = r.d->g.NY;
+    w.f = row_flags<NR, WY>(r);
+    w.vel.load(r);
+    p.mo.load(r.d->buf[a.rMo], r.i0 - 1, r.j0 - 1, NX, NY);
+    p.mf.load(r.d->buf[a.rMf], r.i0 - HI, r.j0 - HJ, NX, NY);
+  }
+  static __device__ __forceinline__ void store(const Pre& p, Lds& l) {
+    p.mo.store(l.Mo);
+    p.mf.store(l.Mf);
+  }
+  static __device__ __forceinline__ void compute(const TileRef& r, const Args& a, const Lds& l, const Rows& w) {
+    const SimDesc* d = r.d;
+    const Grid g = d->g;
+    const int NX = g.NX, NY = g.NY, n = NY;
+    const float dt = a.dt;
+    const Win<R1, C1> wM{l.Mo, r.i0 - 1, r.j0 - 1};
+    const WinSrc<R, C> srcM{Win<R, C>{l.Mf, r.i0 - HI, r.j0 - HJ}, d->buf[a.rMf]};
+    float* Mc = d->buf[a.rMc];
+    const int j = r.j0 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      const int i = r.i0 + threadIdx.y + k * WY;
+      if (i >= NX || j >= NY) continue;
+      const int c = i * n + j;
+      const bool ring = i == 0 || i == NX - 1 || j == 0 || j == NY - 1;
+      const uint32_t f = (w.f >> (8 * k)) & 0xffu;
+      const bool adv = !ring && (f & CF_FLUID);
+      const float om = wM.at(i, j);
+      float u = (w.vel.u0[k] + w.vel.u1[k]) * 0.5f * a.smokeAdv;
+      float v = (w.vel.v0[k] + w.vel.v1[k]) * 0.5f * a.smokeAdv;
+      const NbRange nr = neighbor_range(wM, i, j);  // used at interior cells only
+      int badl = 0;
+      float x = (float)i * g.h + g.hs + dt * u;
+      float y = (float)j * g.h + g.hs + dt * v;
+      const float bs = g.precise ? sample_cells(i, j, (dt * u) * g.h1, (dt * v) * g.h1, srcM, true, true, g, &badl)
+                                 : sample_field(x, y, srcM, g.hs, g.hs, g, &badl);
+      const float bm = adv ? bs : 0.0f;  // solid interior cells: the backward buffer's zero
+      float cm = clamp_to(om - (bm - om) * 0.5f, nr);
+      if (cm < 0.0f) cm = 0.0f;
+      st_cell(Mc, c, ring ? om : cm);
+      if (adv && badl) atomicOr(d->ctrl + 2, 1);
+    }
+  }
+};
+template <class M, int TI, int WY> __global__ void __launch_bounds__(BX* WY) k_bfecc_smoke_corr(M m, CorrSmokeArgs a, int ntiles) {
+  run_tile<M, CorrSmokePass<TI, WY>>(m, a, ntiles);
 }
 int launch_bfecc_smoke_corr(Ctx* c, const float* Morig, const float* Mfwd, float* Mcorr, float dt) {
-  const int a = role_of(c, Morig), b = role_of(c, Mfwd), e = role_of(c, Mcorr);
-  const float sa = c->prm.smoke_advection;
-  launch_cells(c, [&](const OneSim& m, dim3 g, dim3 bl, hipStream_t s) {
-    hipLaunchKernelGGL(k_bfecc_smoke_corr<OneSim>, g, bl, 0, s, m, a, b, e, dt, sa);
-  });
+  const CorrSmokeArgs a{role_of(c, Morig), role_of(c, Mfwd), role_of(c, Mcorr), dt, c->prm.smoke_advection};
+  FL_ADV_LAUNCH(k_bfecc_smoke_corr, c, a);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
@@ -641,26 +985,32 @@ int batch_borders(Batch* B, int rU, int rV) {
   return FL_OK;
 }
 int batch_advect_velocity(Batch* B, int rUt, int rVt, int rUo, int rVo, float dt, bool write_new) {
-  hipLaunchKernelGGL(k_advect_velocity<ManySims>, FL_BATCH_CELLS(B), many(B), rUt, rVt, rUo, rVo, dt,
-                     (int)write_new);
+  const int nt = B->ncell_items;
+  hipLaunchKernelGGL((k_advect_velocity<ManySims, BY, BY>), dim3(nt),
+                     dim3(BX, BY), 0, B->stream, many(B), AdvVelArgs{rUt, rVt, rUo, rVo, dt, (int)write_new}, nt);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 int batch_bfecc_velocity_corr(Batch* B, int rUo, int rVo, int rUf, int rVf, float dt) {
-  hipLaunchKernelGGL(k_bfecc_velocity_corr<ManySims>, FL_BATCH_CELLS(B), many(B), rUo, rVo, rUf, rVf, (int)B_T0,
-                     (int)B_T1, dt);
+  const int nt = B->ncell_items;
+  hipLaunchKernelGGL((k_bfecc_velocity_corr<ManySims, BY, BY>), dim3(nt),
+                     dim3(BX, BY), 0, B->stream, many(B), CorrVelArgs{rUo, rVo, rUf, rVf, B_T0, B_T1, dt}, nt);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 int batch_advect_smoke(Batch* B, int rMs, int rMo, float dt, bool write_new) {
-  hipLaunchKernelGGL(k_advect_smoke<ManySims>, FL_BATCH_CELLS(B), many(B), rMs, rMo, dt, B->prm.smoke_advection,
-                     B->prm.viscosity_diffusion, (int)write_new);
+  const int nt = B->ncell_items;
+  hipLaunchKernelGGL((k_advect_smoke<ManySims, BY, BY>), dim3(nt), dim3(BX, BY), 0,
+                     B->stream, many(B),
+                     AdvSmokeArgs{rMs, rMo, dt, B->prm.smoke_advection, B->prm.viscosity_diffusion, (int)write_new}, nt);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }
 int batch_bfecc_smoke_corr(Batch* B, float dt) {
-  hipLaunchKernelGGL(k_bfecc_smoke_corr<ManySims>, FL_BATCH_CELLS(B), many(B), (int)B_M, (int)B_M2, (int)B_T2, dt,
-                     B->prm.smoke_advection);
+  const int nt = B->ncell_items;
+  hipLaunchKernelGGL((k_bfecc_smoke_corr<ManySims, BY, BY>), dim3(nt),
+                     dim3(BX, BY), 0, B->stream, many(B), CorrSmokeArgs{B_M, B_M2, B_T2, dt, B->prm.smoke_advection},
+                     nt);
   FL_HIP(hipGetLastError());
   return FL_OK;
 }

@@ -63,7 +63,7 @@ def measured_traffic(kind, size, iters):
     return None
 
 
-def cpu_baseline(kind, size, iters, threads):
+def cpu_baseline(kind, size, iters, threads, early_exit=True):
     """The float32 oracle stepping the same scene on this host's cores: a bounded sample,
     up to three steps (at most ~25 s) of the workload at the largest power-of-two size whose
     step is expected to take under ~20 s (the serial Gauss-Seidel sweeps cost ~15 ns per
@@ -73,6 +73,7 @@ def cpu_baseline(kind, size, iters, threads):
     while sample_size > 256 and (sample_size ** 2) * iters * 15e-9 > 20.0:
         sample_size //= 2
     f = build(kind, lambda d, w, h, hh: OracleFluid(d, w, h, hh, threads=threads), sample_size, iters)
+    f.EarlyExit = early_exit
     times = []
     while len(times) < 3 and sum(times) < 25.0:  # 10-30 s of CPU work, median step
         t0 = time.perf_counter()
@@ -531,7 +532,7 @@ def main():
     f.close()
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (float32 oracle, %d threads)" % args.cpu_threads, file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.config, args.size, args.iters, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(args.config, args.size, args.iters, args.cpu_threads, out["early_exit"])
     if args.config == "cavity" and not args.no_extras:
         out.update(metric_extras(args))
     print(json.dumps(out), flush=True)

@@ -364,6 +364,7 @@ void free_ctx(Ctx* c) {
     if (c->flags) (void)hipFree(c->flags);
     if (c->flags_diag) (void)hipFree(c->flags_diag);
     if (c->keep) (void)hipFree(c->keep);
+    if (c->snap_hint) (void)hipHostFree(c->snap_hint);
     if (c->d_ctrl) (void)hipFree(c->d_ctrl);
     if (c->d_maxdiv) (void)hipFree(c->d_maxdiv);
   }
@@ -548,6 +549,9 @@ int fl_create(float density, int width, int height, float h, fl_ctx** out) {
   if ((e = hipMalloc(&c->flags, c->N)) != hipSuccess) return fail(e, "hipMalloc flags");
   if ((e = hipMalloc(&c->keep, sizeof(uint32_t) * c->NX * ((c->NY + 63) / 64))) != hipSuccess)
     return fail(e, "hipMalloc keep");
+  if ((e = hipHostMalloc((void**)&c->snap_hint, 64, hipHostMallocCoherent)) != hipSuccess)
+    return fail(e, "hipHostMalloc hint");
+  *c->snap_hint = 0;
   if ((e = hipMalloc(&c->flags_diag, (size_t)(c->NX + c->NY - 1) * c->NX)) != hipSuccess)
     return fail(e, "hipMalloc flags_diag");
   // entries off the grid stay zero for the context's lifetime

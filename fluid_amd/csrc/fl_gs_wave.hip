@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fl_internal.h"
@@ -119,7 +120,10 @@ __device__ __forceinline__ const SimDesc& sim_desc(const OneSim& m, int) { retur
 __device__ __forceinline__ const SimDesc& sim_desc(const ManySims& m, int e) { return m.sims[e]; }
 
 // TK sweeps per tile (one wave each), BLK columns per staging and publish block (BLK divides 64).
-template <int TK, int BLK, class M>
+// SNAP: the main run also writes the first sweep's U, V, p to T0, T1, T2, so that an early exit
+// after sweep 0 needs no re-run (k_gs_take_first); a separate instantiation, so the usual kernel
+// carries none of its code or registers.
+template <int TK, int BLK, class M, bool SNAP = false>
 __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
   static_assert(LANES % BLK == 0, "progress is published in whole blocks that divide the tile height");
   const int lane = threadIdx.x & (LANES - 1);
@@ -172,6 +176,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     const bool wave_on = k < K;
     // wave-uniform in a scalar register: the other waves skip the output block with one branch
     const int last = __builtin_amdgcn_readfirstlane((int)(k == K - 1));
+    const int first_out = SNAP ? __builtin_amdgcn_readfirstlane((int)(R.which == 0 && k == 0)) : 0;
     const int i0 = LANES * g - k0;           // row of (lane 0, wave 0)
     const int row = i0 + lane - wave;        // this thread's row
     const bool real_row = row >= 0 && row <= NX - 1;
@@ -459,6 +464,15 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
             if (tp) Pout[c] = p;
           }
         }
+        if (first_out) {  // the first sweep's result, kept for an early exit after it
+          const int j = s - lane - wave;
+          if (real_row && j >= 0 && j <= NY - 1) {
+            const size_t c = (size_t)row * NY + j;
+            if (row >= 1 && j >= 1 && j <= NY - 2) D.buf[B_T0][c] = Ul;
+            if (row >= 1 && row <= NX - 2 && j >= 1) D.buf[B_T1][c] = Vb;
+            if (row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2) D.buf[B_T2][c] = p;
+          }
+        }
         __syncthreads();
       }
       if (more) {
@@ -518,7 +532,7 @@ __global__ void k_gs_init(int32_t* prog, int nprog, uint32_t* maxdiv, int nmd, i
   const int stride = blockDim.x * gridDim.x;
   for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nprog; t += stride) prog[t] = 0;
   for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nmd; t += stride) maxdiv[t] = 0u;
-  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nsim; t += stride) ctrl[4 * t + 1] = 0;
+  for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < nsim; t += stride) ctrl[4 * t] = ctrl[4 * t + 1] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     ticket[0] = 0;  // main run
     ticket[1] = 0;  // re-run
@@ -528,9 +542,14 @@ __global__ void k_gs_init(int32_t* prog, int nprog, uint32_t* maxdiv, int nmd, i
 
 // Early exit of solveSingleGrid (fluid.go:175-177), one wave per simulation: the first
 // sweep whose max |div| is below 1e-5 ends the solve; if that is not the last sweep,
-// request a re-run of k*+1 sweeps (and flag the launch, rerun[0], for the re-run kernel).
+// request a re-run of k*+1 sweeps (and flag the launch, rerun[0], for the re-run kernel),
+// or, when the main run kept the first sweep's result (snap) and k* = 0, mark it to be
+// taken instead (ctrl[0], k_gs_take_first). ctrl[1] holds the kept sweeps either way.
+// hint (host memory, may be null): whether this solve ended after sweep 0, which decides
+// whether the next solve keeps the first sweep's result.
 // Lane l scans sweeps l, l+64, ...; the wave's minimum is the first such sweep.
-template <class M> __global__ void k_gs_decide(M m, int nsim, int N, int32_t* rerun) {
+template <class M>
+__global__ void k_gs_decide(M m, int nsim, int N, int32_t* rerun, int snap, int32_t* hint) {
   const int e = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int lane = threadIdx.x & (LANES - 1);
   if (e >= nsim) return;
@@ -545,9 +564,29 @@ template <class M> __global__ void k_gs_decide(M m, int nsim, int N, int32_t* re
   for (int o = LANES / 2; o > 0; o >>= 1) first = min(first, __shfl_xor(first, o, LANES));
   if (lane == 0) {
     const int kept = first < N ? first + 1 : N;
+    const bool take = snap && kept == 1 && kept < N;
+    D.ctrl[0] = take ? 1 : 0;
     D.ctrl[1] = kept < N ? kept : 0;
-    if (kept < N) atomicOr(rerun, 1);
+    if (kept < N && !take) atomicOr(rerun, 1);
+    if (hint) *hint = kept == 1 && kept < N ? 1 : 0;
   }
+}
+
+// An early exit after sweep 0 with the first sweep's result kept: copy it into the solve's
+// outputs (the faces and p the sweeps write; the ring is already there) and its max |div| into
+// the re-run's record, which is where a re-run of one sweep would have put it.
+template <class M> __global__ void k_gs_take_first(M m, int rPout) {
+  const SimDesc& D = sim_desc(m, 0);
+  if (D.ctrl[0] == 0) return;
+  const int NX = D.g.NX, NY = D.g.NY;
+  const size_t n = (size_t)NX * NY;
+  for (size_t c = threadIdx.x + (size_t)blockIdx.x * blockDim.x; c < n; c += (size_t)gridDim.x * blockDim.x) {
+    const int row = (int)(c / NY), j = (int)(c % NY);
+    if (row >= 1 && j >= 1 && j <= NY - 2) D.buf[B_U2][c] = D.buf[B_T0][c];
+    if (row >= 1 && row <= NX - 2 && j >= 1) D.buf[B_V2][c] = D.buf[B_T1][c];
+    if (row >= 1 && row <= NX - 2 && j >= 1 && j <= NY - 2) D.buf[rPout][c] = D.buf[B_T2][c];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) D.maxdiv[D.nmax] = D.maxdiv[0];
 }
 
 static bool tile_empty_h(int g, int kb, int K, int TK, int NX) {
@@ -612,9 +651,14 @@ static long long tiles_at16(int NX, int N) { return (long long)((NX + N - 1 + 63
 // many workgroups as fit on the device at once (and no more than there are tiles).
 template <class M> struct GsKernels {
   using Fn = void (*)(M, GsRun);
-  static Fn get(int tk, int blk) {
-#define FL_GS_CASE(T, B) \
-  if (tk == T && blk == B) return k_gs_wave<T, B, M>;
+  // snap: the instantiation that keeps the first sweep's result (single simulations only)
+  static Fn get(int tk, int blk, bool snap = false) {
+#define FL_GS_CASE(T, B)                               \
+  if (tk == T && blk == B) {                           \
+    if constexpr (std::is_same<M, OneSim>::value)      \
+      if (snap) return k_gs_wave<T, B, M, true>;       \
+    return snap ? nullptr : k_gs_wave<T, B, M, false>; \
+  }
     FL_GS_CASE(2, 4) FL_GS_CASE(2, 8) FL_GS_CASE(2, 16)
     FL_GS_CASE(4, 4) FL_GS_CASE(4, 8) FL_GS_CASE(4, 16)
     FL_GS_CASE(8, 4) FL_GS_CASE(8, 8) FL_GS_CASE(8, 16)
@@ -630,8 +674,9 @@ template <class M> static int occupancy_grid(int dev, int tk, int blk, int ntile
   (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, GsKernels<M>::get(tk, blk), tk * LANES, 0);
   return std::max(1, std::min(ntiles, std::max(1, per_cu) * prop.multiProcessorCount));
 }
-template <class M> static int launch_gs(int tk, int blk, int grid, hipStream_t stream, const M& m, const GsRun& R) {
-  auto fn = GsKernels<M>::get(tk, blk);
+template <class M>
+static int launch_gs(int tk, int blk, int grid, hipStream_t stream, const M& m, const GsRun& R, bool snap = false) {
+  auto fn = GsKernels<M>::get(tk, blk, snap);
   if (!fn) {
     set_error("no Gauss-Seidel kernel for " + std::to_string(tk) + " sweeps x " + std::to_string(blk) + " columns");
     return FL_ERR_INTERNAL;
@@ -691,7 +736,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, 2 * nprog, c->d_maxdiv, 2 * c->nmax,
                      c->d_ctrl, 1, W.d_ticket);
   FL_HIP(hipGetLastError());
-  GsRun R;
+  GsRun R{};
   R.tiles = (const int2*)W.d_tile;
   R.ntiles = W.ntiles;
   R.K = N;
@@ -702,14 +747,24 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.defer = defer_handoff(c->prm, W.ntiles, W.grid);
   // without the early exit no re-run can be requested: the decision and re-run are skipped
   const int runs = c->prm.early_exit ? 2 : 1;
+  // Keep the first sweep's result (in T0-T2, scratch during the solve) when the last solve that
+  // reported back ended after sweep 0: a quiescent field then takes it instead of a re-run
+  // launch that costs as much as the main one on small grids.
+  const bool snap = runs > 1 && N > 1 && c->snap_hint && __atomic_load_n(c->snap_hint, __ATOMIC_RELAXED) != 0;
   for (int which = 0; which < runs; which++) {
     R.which = which;
     R.ticket = W.d_ticket + which;
     R.rerun = which ? W.d_ticket + 2 : nullptr;
-    if ((rc = launch_gs(W.tk, W.blk, W.grid, c->stream, m, R))) return rc;
+    if ((rc = launch_gs(W.tk, W.blk, W.grid, c->stream, m, R, snap && which == 0))) return rc;
     if (which == 0 && runs > 1) {
-      hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(LANES), 0, c->stream, m, 1, N, W.d_ticket + 2);
+      hipLaunchKernelGGL(k_gs_decide<OneSim>, dim3(1), dim3(LANES), 0, c->stream, m, 1, N, W.d_ticket + 2,
+                         (int)snap, N > 1 ? c->snap_hint : nullptr);
       FL_HIP(hipGetLastError());
+      if (snap) {
+        const int nb = (int)std::min<size_t>(1024, (c->N + 255) / 256);
+        hipLaunchKernelGGL(k_gs_take_first<OneSim>, dim3(nb), dim3(256), 0, c->stream, m, (int)B_T3);
+        FL_HIP(hipGetLastError());
+      }
     }
   }
   std::swap(c->U, c->U2);
@@ -819,7 +874,7 @@ int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
   hipLaunchKernelGGL(k_gs_init, dim3(256), dim3(256), 0, B->stream, B->prog, B->gs_nprog, B->maxdiv_pool,
                      2 * B->nmax * E, B->ctrl_pool, E, B->ticket);
   FL_HIP(hipGetLastError());
-  GsRun R;
+  GsRun R{};
   R.tiles = B->d_tiles;
   R.ntiles = B->ntiles;
   R.K = N;
@@ -838,7 +893,7 @@ int batch_gs_solve(Batch* B, int N, int rPin, int rPout) {
     if (which == 0 && runs > 1) {
       // one wave per member, four per workgroup
       hipLaunchKernelGGL(k_gs_decide<ManySims>, dim3((E + 3) / 4), dim3(4 * LANES), 0, B->stream, m, E, N,
-                         B->ticket + 2);
+                         B->ticket + 2, 0, nullptr);
       FL_HIP(hipGetLastError());
     }
   }

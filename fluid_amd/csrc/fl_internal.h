@@ -135,6 +135,7 @@ struct Ctx {
   float *T0 = nullptr, *T1 = nullptr, *T2 = nullptr, *T3 = nullptr;  // BFECC / curl scratch
   uint8_t* flags = nullptr;      // cell flags, i-major
   uint32_t* keep = nullptr;      // SimDesc::keep
+  int32_t* snap_hint = nullptr;  // host-pinned: the last solve that reported back ended after sweep 0
   uint8_t* flags_diag = nullptr; // cell flags, anti-diagonal-major [d=i+j][i]
   bool flags_dirty = true;
 

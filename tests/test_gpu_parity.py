@@ -124,6 +124,31 @@ def test_simulate_bit_exact(solver, opts):
     assert g.stats()["sweeps_run"] == o.last_sweeps()
 
 
+@pytest.mark.parametrize("shape", [(4, 8), (8, 8), (16, 4)], ids=["tk4", "tk8", "tk16_blk4"])
+def test_early_exit_after_first_sweep_repeated(shape):
+    """Steps whose solve ends after sweep 0, one after another: once a solve has ended there,
+    the next main run also keeps its first sweep's result and takes it instead of a re-run.
+    Every step is bit-exact with the oracle and reports one kept sweep below the tolerance."""
+    g, o = pair(1000.0, 70, 50, 0.01)
+    st = {k: np.zeros((72, 52), np.float32) for k in FIELDS}
+    st["S"] = np.ones((72, 52), np.float32)
+    st["S"][0, :] = st["S"][:, 0] = st["S"][:, -1] = 0
+    load_both(g, o, st)
+    for f in (g, o):
+        knobs(f, PressureIters=20)
+    g.BlockSweeps, g.BlockCols = shape
+    for step in range(5):
+        U = np.zeros((72, 52), np.float32)
+        U[30 + step, 20] = 2e-8 * (step + 1)  # divergence below the tolerance after one sweep
+        for f in (g, o):
+            f.set("U", U)
+            f.Simulate(0.1)
+        assert_same(g, o, "step %d" % step)
+        s = g.stats()  # also waits for the step, so the next solve sees how this one ended
+        assert s["sweeps_run"] == o.last_sweeps() == 1 and s["reran"], (step, s)
+        assert 0.0 <= s["max_div_last"] < 1e-5, s
+
+
 @pytest.mark.parametrize("solver", [1, 2], ids=["hyper", "wave"])
 @pytest.mark.parametrize("lag", [0, 2], ids=["auto_lag", "late_handoff"])
 def test_early_exit_rollback(solver, lag):

@@ -503,7 +503,10 @@ def main():
         "roofline": None if args.multigrid else {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic(args.config, args.size, args.iters),
-            "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells)},
+            "kernel": "pressure solve (%d sweeps x 28 B x %d cells per launch)" % (sweeps, cells),
+            "note": "achieved counts the algorithmic bytes of a sweep-by-sweep solve; the temporal-blocked "
+                    "kernel moves `traffic` bytes of HBM per launch (rate in hbm_measured) and is bound by its "
+                    "critical path of column steps and tile handoffs, not by HBM"},
         "early_exit": bool(f.EarlyExit),
         "step_alg_bytes_per_cell": b_alg,
         "step_achieved_GBs": step_gbs,

@@ -156,8 +156,6 @@ static int prepare_batch(Batch* B, float dt) {
   int rc;
   const int N = B->prm.pressure_iters;
   if (N > MEMBER_NMAX) return berr(FL_ERR_UNSUPPORTED, "batched solves support at most 1024 sweeps");
-  if (B->prm.use_multigrid && B->prm.multigrid_levels > 1)
-    return berr(FL_ERR_UNSUPPORTED, "UseMultigrid: the multigrid V-cycle is not implemented on the device");
   if ((rc = upload_omega(B, N))) return rc;
   // descriptors of all members with this step's cp and the GS layout for N sweeps
   std::vector<SimDesc> d(B->members.size());
@@ -240,7 +238,25 @@ static void swap_member_roles(Batch* B) {
   }
 }
 
+// UseMultigrid (fluid.go:148-151): the V-cycle decides on the host between cycles whether to go
+// on, which neither the merged persistent launch nor graph capture allows, so each member takes
+// its own update (its sources, then Simulate, on the batch's stream) with the batch's knobs. The
+// members' buffers trade places as in a member's own step; the next batched step reads their
+// new descriptors.
+static int batch_update_members(Batch* B, float dt) {
+  for (Ctx* c : B->members) {
+    const fl_params saved = c->prm;
+    c->prm = B->prm;
+    const int rc = fl_scene_update(reinterpret_cast<fl_ctx*>(c), dt);
+    c->prm = saved;
+    if (rc) return rc;
+  }
+  B->steps++;
+  return FL_OK;
+}
+
 static int batch_update(Batch* B, float dt) {
+  if (B->prm.use_multigrid && B->prm.multigrid_levels > 1) return batch_update_members(B, dt);
   int rc = prepare_batch(B, dt);
   if (rc) return rc;
   if (B->use_graph) {
@@ -490,7 +506,7 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
   if (!b) return berr(FL_ERR_INVALID_ARG, "null batch");
   Batch* B = B_OF(b);
   const int N = B->prm.pressure_iters;
-  if (N <= 0) return FL_OK;
+  if (N <= 0 || (B->prm.use_multigrid && B->prm.multigrid_levels > 1)) return FL_OK;
   if (hipSetDevice(B->dev) != hipSuccess) return berr(FL_ERR_HIP, "hipSetDevice");
   long long tiles8 = 0;
   for (Ctx* c : B->members) tiles8 += tiles_at(c->NX, N, 8);

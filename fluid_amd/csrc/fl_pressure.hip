@@ -378,10 +378,6 @@ int multigrid_solve(Ctx* c, int numIters, float dt) {
   c->last_n = 0;
   c->host_sweeps = 0;
   if (numIters <= 0) return FL_OK;
-  if (c->batch) {
-    set_error("UseMultigrid is not supported for batch members");
-    return FL_ERR_UNSUPPORTED;
-  }
   int rc = ensure_omega(c, std::max(numIters, 3));
   if (rc) return rc;
   const int cNX = (c->NX + 1) / 2, cNY = (c->NY + 1) / 2;

@@ -118,6 +118,41 @@ def test_batch_every_shape_matches_solo(shape):
         s.close()
 
 
+def test_batch_multigrid_members_match_solo_and_oracle():
+    """UseMultigrid on a batch: every member takes the V-cycle in its own update with the
+    batch's knobs, bit-identical to stepping it alone and to the oracle; switching back to the
+    merged single-grid solve afterwards follows the members' traded buffers."""
+    sizes = [(48, 40), (130, 70), (64, 64), (33, 77)]
+    knobs = dict(UseBFECC=True, PressureIters=4, EarlyExit=True)
+    b, solo = make_pair(sizes, knobs)
+    o = OracleFluid(C.DENSITY, 33, 77, C.H)
+    scene(3, o)
+    for k, v in dict(BASE_KNOBS, **knobs).items():
+        if k not in ("BlockSweeps", "HandoffLag"):
+            setattr(o, k, v)
+    for f in [b, o] + solo:
+        f.UseMultigrid, f.MultigridLevels = True, 2
+    for step in range(3):
+        b.update(C.DT)
+        for s in solo:
+            s.update(C.DT)
+        o.update(C.DT)
+        for e, (m, s) in enumerate(zip(b.members, solo)):
+            same(m, s, "multigrid step %d member %d %s" % (step, e, sizes[e]))
+        same(b.members[3], o, "multigrid step %d oracle" % step)
+    for f in [b] + solo:
+        f.UseMultigrid, f.PressureIters = False, 20
+    for step in range(2):
+        b.update(C.DT)
+        for s in solo:
+            s.update(C.DT)
+        for e, (m, s) in enumerate(zip(b.members, solo)):
+            same(m, s, "single-grid step %d member %d" % (step, e))
+    b.close()
+    for s in solo:
+        s.close()
+
+
 def test_batch_member_vs_oracle():
     """One ensemble member against the oracle directly (no solo Fluid in between)."""
     sizes = [(90, 50), (33, 77)]

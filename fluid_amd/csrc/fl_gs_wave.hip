@@ -284,7 +284,8 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         }
       }
     };
-    // Flags of this thread's cells for the BLK columns of a block (anti-diagonal layout).
+    // Flags of this thread's cells for the BLK columns of a block (anti-diagonal layout), one
+    // byte load each into its own register.
     auto load_flags = [&](int blk, uint32_t* fl) {
 #pragma unroll
       for (int u = 0; u < BLK; u++) {
@@ -293,6 +294,25 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         uint32_t f = 0;
         if (wave_on && real_row && dd >= 0 && dd <= NX + NY - 2) f = fdiag[(size_t)dd * NX + row];
         fl[u] = f;
+      }
+    };
+    // The flags of the block being run: with 8-sweep tiles packed four to a register at the
+    // block's end (when the loads have returned; packing them as they arrive would make the wave
+    // wait for them, and for every load issued before them, at the block's start). That holds
+    // the 8x8 kernel to 119 VGPRs, two workgroups per CU instead of one: 4096^2 cavity 12.42 ->
+    // 11.89 ms/step. The unpacking costs a bit-field extract per step, which loses 5% where it
+    // does not change the occupancy (16-sweep tiles), so other shapes keep one flag a register.
+    constexpr bool PACK = TK == 8;
+    constexpr int FW = PACK ? (BLK + 3) / 4 : BLK;
+    auto take_flags = [&](const uint32_t* fl, uint32_t* cur) {
+      if constexpr (PACK) {
+#pragma unroll
+        for (int w = 0; w < FW; w++) cur[w] = 0;
+#pragma unroll
+        for (int u = 0; u < BLK; u++) cur[u / 4] |= fl[u] << (8 * (u % 4));
+      } else {
+#pragma unroll
+        for (int u = 0; u < BLK; u++) cur[u] = fl[u];
       }
     };
     // Producer side, in two halves run by the last wave (which also stores the K-stream
@@ -344,8 +364,13 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
     __syncthreads();
     load_block(0);
     store_block(0);
-    uint32_t fcur[BLK], fnext[BLK];
-    load_flags(0, fcur);
+    uint32_t fcur[FW], fnext[BLK];
+    if constexpr (PACK) {
+      load_flags(0, fnext);
+      take_flags(fnext, fcur);
+    } else {
+      load_flags(0, fcur);
+    }
     __syncthreads();
 
     float UrOut = 0.f, Vreg = 0.f, pn = 0.f, md = 0.f;
@@ -395,7 +420,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
         float p = pn;
         pn = pnew;
         // ---- the reference cell update (fluid.go:196-229)
-        const uint32_t f = fcur[u];
+        const uint32_t f = PACK ? (fcur[u / 4] >> (8 * (u % 4))) & 0xFFu : fcur[u];
         if (__all((f & 0x3Fu) == 0x3Fu)) {
           // Every lane holds a fluid interior cell whose four neighbours are fluid: s = 4 and
           // all weights are 1, so (-div)/4 is the exact product with 0.25 and the face updates
@@ -477,8 +502,7 @@ __global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
       }
       if (more) {
         store_block(cb ^ 1);
-#pragma unroll
-        for (int u = 0; u < BLK; u++) fcur[u] = fnext[u];
+        take_flags(fnext, fcur);
         __syncthreads();
       }
     }
@@ -625,27 +649,27 @@ static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, 
 static bool shape_ok(int tk, int blk) {
   return (tk == 2 || tk == 4 || tk == 8 || tk == 16) && (blk == 4 || blk == 8 || blk == 16);
 }
-// Sweeps per tile: the knob if set; otherwise 16 when the 16-deep tiling yields enough tiles
-// to keep every CU busy (wider blocks of sweeps halve the handoff traffic), 4 for solves of at
-// most 8 sweeps (Simulate's own count: one wave per SIMD, 300x251 frame 0.466 -> 0.447 ms),
-// else 8 (twice the tiles of 16, shorter pipeline fill). `tiles16` counts the tiles at depth 16.
-int pick_tk(const fl_params& P, int N, long long tiles16) {
+// Sweeps per tile: the knob if set; otherwise 4 for solves of at most 8 sweeps (Simulate's own
+// count: one wave per SIMD, 300x251 frame 0.466 -> 0.447 ms), else 8: twice the tiles of 16 and
+// a shorter pipeline fill, and with its flags packed two workgroups per CU. Measured against 16
+// on the 4096^2 cavity (11.8 against 12.4 ms/step, each with its best publication) and equal on
+// the 256-member ensemble.
+int pick_tk(const fl_params& P, int N) {
   if (P.block_sweeps == 2 || P.block_sweeps == 4 || P.block_sweeps == 8 || P.block_sweeps == 16)
     return P.block_sweeps;
-  if (N >= 64 && tiles16 >= 512) return 16;
   return N <= 8 ? 4 : 8;
 }
 // Columns per block: the knob if set, else 8.
 int pick_blk(const fl_params& P) { return (P.block_cols == 4 || P.block_cols == 16) ? P.block_cols : 8; }
 // When handoff rows are published: on time (handoff_lag 1: GsRun.defer 0), one block late
-// (2: defer 1) or halfway through the next block (3: defer 2); automatic (0) picks one block
-// late when the tiles fill the persistent grid at least twice (the solve is throughput-bound
-// and the per-block store wait costs more than the longer pipeline fill), else on time.
+// (2: defer 1) or halfway through the next block (3: defer 2); automatic (0) picks halfway
+// when the tiles fill the persistent grid at least twice (the solve is throughput-bound and the
+// per-block store wait costs more than the longer pipeline fill: 4096^2 cavity 12.10 ms/step on
+// time, 12.15 one block late, 11.89 halfway), else on time.
 static int defer_handoff(const fl_params& P, int ntiles, int grid) {
   if (P.handoff_lag >= 1 && P.handoff_lag <= 3) return P.handoff_lag - 1;
-  return ntiles >= 2 * grid ? 1 : 0;
+  return ntiles >= 2 * grid ? 2 : 0;
 }
-static long long tiles_at16(int NX, int N) { return (long long)((NX + N - 1 + 63) / 64) * ((N + 15) / 16); }
 
 // The kernel of one tile shape and launch map, and its persistent grid for `ntiles` tiles: as
 // many workgroups as fit on the device at once (and no more than there are tiles).
@@ -688,7 +712,7 @@ static int launch_gs(int tk, int blk, int grid, hipStream_t stream, const M& m, 
 
 static int layout(Ctx* c, int N) {
   GsWork& W = c->gs;
-  const int tk = pick_tk(c->prm, N, tiles_at16(c->NX, N));
+  const int tk = pick_tk(c->prm, N);
   const int blk = pick_blk(c->prm);
   if (W.n_alloc == N && W.tk == tk && W.blk == blk) return FL_OK;
   if (!shape_ok(tk, blk)) return FL_ERR_INTERNAL;
@@ -791,9 +815,7 @@ int pressure_free(Ctx* c) {
 // rPin (rPin < 0: all zero, Simulate's fill(p, 0)), the solve's p written to rPout, the
 // velocity to U2, V2.
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
-  long long t16 = 0;
-  for (const SimDesc& d : descs) t16 += tiles_at16(d.g.NX, N);
-  const int tk = pick_tk(B->prm, N, t16);
+  const int tk = pick_tk(B->prm, N);
   const int blk = pick_blk(B->prm);
   if (!shape_ok(tk, blk)) return FL_ERR_INTERNAL;
   auto fill = [&]() {

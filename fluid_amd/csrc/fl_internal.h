@@ -347,7 +347,7 @@ int batch_apply_sources(Batch* B);
 int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const int32_t* seg, const int32_t* segsim,
                       int nseg);
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
-int pick_tk(const fl_params& P, int N, long long tiles16);
+int pick_tk(const fl_params& P, int N);
 int pick_blk(const fl_params& P);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
 int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p from rPin (< 0: zero), to rPout

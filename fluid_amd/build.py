@@ -13,6 +13,12 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfluidhip.so")
+# The same library with AddressSanitizer on its host code (contexts, batches, argument checks,
+# staged copies, the tuner): a debug build for the host runtime. Each -fsanitize= follows
+# -Xarch_host, so the device code is built exactly as in the release library. A process loads
+# it with the ASan runtime preloaded (asan_runtime()).
+ASAN_LIB = os.path.join(HERE, "libfluidhip_asan.so")
+ASAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
 SOURCES = ["fl_capi.hip", "fl_kernels.hip", "fl_pressure.hip", "fl_gs_wave.hip", "fl_batch.hip", "fl_adjoint.hip", "fl_render.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -48,19 +54,39 @@ def _run(cmd, verbose):
         print(out)
 
 
-def build(verbose=False):
-    tmp = LIB + ".tmp"
+def build(verbose=False, asan=False):
+    """Compile and link the library (asan: the host-sanitized debug build, ASAN_LIB)."""
+    out = ASAN_LIB if asan else LIB
+    extra = ASAN_FLAGS if asan else []
+    tmp = out + ".tmp"
     with tempfile.TemporaryDirectory() as d:
         def compile_one(src):
             obj = os.path.join(d, src + ".o")
-            _run([HIPCC] + FLAGS + PER_SOURCE.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj], verbose)
+            _run([HIPCC] + FLAGS + extra + PER_SOURCE.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj],
+                 verbose)
             return obj
         with ThreadPoolExecutor(min(len(SOURCES), os.cpu_count() or 1)) as ex:
             objs = list(ex.map(compile_one, SOURCES))
+        # the sanitized objects are linked like the release ones: their __asan_* references
+        # resolve against the runtime the process preloads
         _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
+
+
+def asan_runtime():
+    """Path of the AddressSanitizer runtime to preload for ASAN_LIB: GCC's libasan. hipcc's own
+    runtime also intercepts the HSA runtime's device allocations to pad them for device-side
+    ASan, and those allocations fail (the HIP runtime's first 4 MB pool allocation reports out of
+    memory); the host-only build needs none of that. GCC's runtime implements the same ASan ABI
+    (version 8; it exports every __asan_* symbol the sanitized objects reference) without the
+    HSA interceptors."""
+    r = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True)
+    path = os.path.realpath(r.stdout.strip()) if r.returncode == 0 else ""
+    if not os.path.isabs(path) or not os.path.exists(path):
+        raise RuntimeError("GCC's AddressSanitizer runtime (libasan.so) not found")
+    return path
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, asan="--asan" in sys.argv))

@@ -523,6 +523,8 @@ def main():
                        "reproduces bit for bit) max|u| reaches 10 by step 9, 1e13 by step 22 and NaN at step 24 at "
                        "2048x1024; use --steps 8 --warmup 2 to time the steps before it runs away"
                        % (args.warmup + 1, args.warmup + args.steps))
+    if out["roofline"]:
+        out["latency_roof"] = latency_roof(f, solve_ms)
     traffic = out["roofline"]["traffic"] if out["roofline"] else None
     if traffic:
         # the solve's physical HBM traffic (rocprof FETCH_SIZE + WRITE_SIZE per launch, calibrated
@@ -539,6 +541,73 @@ def main():
     if args.config == "cavity" and not args.no_extras:
         out.update(metric_extras(args))
     print(json.dumps(out), flush=True)
+
+
+def critical_path_steps(L):
+    """Column steps on the solve's longest chain of dependent tiles, from its tiling (a dict of
+    fl_get_gs_layout). The kernel's wait rule: before running block b a tile stages block b + 1,
+    so it needs its G producer (the tile one row-tile up) to have published row s1 + 64 and its
+    K producer (the tile one sweep-block back) row s1 + TK, s1 being block b + 1's last step.
+    Rows are published in whole blocks (on time: when the next block starts; halfway: BLK/2 steps
+    later; one block late: BLK steps later), which puts a consumer 2 + ceil((L - 1) / BLK) blocks
+    plus that delay behind its producer, L = 64 along the rows and TK along the sweeps. The last
+    tile starts after G - 1 row hops and KB - 1 sweep hops, then runs its own steps."""
+    tk, blk, lag = L["sweeps_per_tile"], L["cols_per_block"], L["handoff_lag"]
+    delay = {1: 0, 2: blk, 3: blk // 2}[lag]
+
+    def hop(span):
+        return (2 + -(-(span - 1) // blk)) * blk + delay
+    return (L["row_tiles"] - 1) * hop(64) + (L["sweep_blocks"] - 1) * hop(tk) + L["steps_per_tile"]
+
+
+def column_step_us(L, dt):
+    """Wall time of one column step of a tile running alone (no producer to wait for): a 40-row
+    strip 4000 columns long, one tile of the same shape and publication, solved 5 times."""
+    import numpy as np
+    from fluid_amd import Fluid
+    f = Fluid(1000.0, 40, 4000, 0.01)
+    rng = np.random.default_rng(7)
+    st = {k: (0.3 * rng.standard_normal((42, 4002))).astype(np.float32) for k in ("U", "V", "newU", "newV")}
+    S = np.ones((42, 4002), np.float32)
+    S[0, :] = S[-1, :] = S[:, 0] = S[:, -1] = 0
+    st["S"] = S
+    f.load_state(st)
+    f.EarlyExit = False
+    f.BlockSweeps, f.BlockCols, f.HandoffLag = L["sweeps_per_tile"], L["cols_per_block"], L["handoff_lag"]
+    tk = L["sweeps_per_tile"]
+    for _ in range(2):
+        f.run_pass("makeIncompressible", dt, tk)
+    f.sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        f.run_pass("makeIncompressible", dt, tk)
+    f.sync()
+    per_solve = (time.perf_counter() - t0) / 5
+    one = f.gs_layout()
+    f.close()
+    assert one["tiles"] == 1, one
+    return 1e6 * per_solve / one["steps_per_tile"]
+
+
+def latency_roof(f, solve_ms):
+    """The temporal-blocked solve's latency bound: its critical path of column steps times the
+    measured cost of one column step of a lone tile. `frac` is that bound over the solve's
+    measured time (how far the solve is from running at the speed of its longest chain)."""
+    from fluid_amd import configs as C
+    try:
+        L = f.gs_layout()
+    except Exception:
+        return None
+    steps = critical_path_steps(L)
+    us = column_step_us(L, C.DT)
+    ms = steps * us * 1e-3
+    return {"critical_path_steps": steps, "column_step_us": us, "ms": ms, "frac": ms / solve_ms,
+            "shape": "%d sweeps x %d columns per tile block, %s publication, %d tiles on %d workgroups" % (
+                L["sweeps_per_tile"], L["cols_per_block"], {1: "on-time", 2: "one-block-late", 3: "halfway"}[
+                    L["handoff_lag"]], L["tiles"], L["workgroups"]),
+            "note": "column_step_us is measured on a single tile (a 40x4000 strip, whose edge waves take "
+                    "the general update); the rest of frac is time tiles spend in the pipeline's fill and "
+                    "drain, waiting on producers and sharing CUs"}
 
 
 def metric_extras(args):

@@ -45,6 +45,19 @@ class FlStats(ctypes.Structure):
     ]
 
 
+class FlGsLayout(ctypes.Structure):
+    _fields_ = [
+        ("sweeps_per_tile", ctypes.c_int32),
+        ("cols_per_block", ctypes.c_int32),
+        ("handoff_lag", ctypes.c_int32),
+        ("row_tiles", ctypes.c_int32),
+        ("sweep_blocks", ctypes.c_int32),
+        ("steps_per_tile", ctypes.c_int32),
+        ("tiles", ctypes.c_int32),
+        ("workgroups", ctypes.c_int32),
+    ]
+
+
 class FlTiming(ctypes.Structure):
     _fields_ = [
         ("total_ms", ctypes.c_double),
@@ -76,6 +89,7 @@ SIGNATURES = {
     "fl_sync": (_i, [_vp]),
     "fl_get_stats": (_i, [_vp, ctypes.POINTER(FlStats)]),
     "fl_sweep_maxdiv": (_i, [_vp, _vp, _i, _ip]),
+    "fl_get_gs_layout": (_i, [_vp, ctypes.POINTER(FlGsLayout)]),
     "fl_set_timing": (_i, [_vp, _i]),
     "fl_get_timing": (_i, [_vp, ctypes.POINTER(FlTiming)]),
     "fl_set_solid": (_i, [_vp, _i, _i, _i]),

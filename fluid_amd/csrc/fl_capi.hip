@@ -750,6 +750,24 @@ int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count) {
   return FL_OK;
 }
 
+int fl_get_gs_layout(fl_ctx* ctx, fl_gs_layout* out) {
+  CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
+  CTX_CHECK(ctx);
+  if (!out) return err(FL_ERR_INVALID_ARG, "null argument");
+  const GsWork& W = C_OF(ctx)->gs;
+  if (W.defer < 0) return err(FL_ERR_INVALID_ARG, "no temporal-blocked solve has run on this context");
+  *out = fl_gs_layout{};
+  out->sweeps_per_tile = W.tk;
+  out->cols_per_block = W.blk;
+  out->handoff_lag = W.defer + 1;
+  out->row_tiles = W.G;
+  out->sweep_blocks = W.KB;
+  out->steps_per_tile = W.rows;
+  out->tiles = W.ntiles;
+  out->workgroups = W.grid;
+  return FL_OK;
+}
+
 int fl_set_solid(fl_ctx* ctx, int i, int j, int solid) {
   CtxErrScope err_scope_(reinterpret_cast<fl::Ctx*>(ctx));
   if (!ctx) return err(FL_ERR_INVALID_ARG, "null context");

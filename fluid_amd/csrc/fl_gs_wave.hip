@@ -769,6 +769,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.rPin = B_P;  // then p and T3 trade places
   R.rPout = B_T3;
   R.defer = defer_handoff(c->prm, W.ntiles, W.grid);
+  W.defer = R.defer;
   // without the early exit no re-run can be requested: the decision and re-run are skipped
   const int runs = c->prm.early_exit ? 2 : 1;
   // Keep the first sweep's result (in T0-T2, scratch during the solve) when the last solve that

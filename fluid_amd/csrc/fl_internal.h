@@ -107,6 +107,7 @@ struct GsWork {
   int rows = 0;            // handoff rows per tile (= steps per tile)
   int ntiles = 0;          // non-empty tiles in ticket order
   int grid = 0;            // persistent workgroups launched
+  int defer = -1;          // handoff publication of the last solve (GsRun.defer; -1: none yet)
   int32_t* d_tile = nullptr;    // [ntiles] packed (g << 16 | kb), ticket order
   float* d_kout = nullptr;      // [G*KB][rows][3][65]  K-stream handoff planes U, V, p
   float* d_gout = nullptr;      // [G*KB][rows][3][tk]  G-stream handoff planes

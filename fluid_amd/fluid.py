@@ -226,6 +226,12 @@ class Fluid:
             check(self._L.fl_sweep_maxdiv(self._ctx, out.ctypes.data, cnt.value, ctypes.byref(cnt)))
         return out
 
+    def gs_layout(self):
+        """The tiling of the last temporal-blocked solve (fl_get_gs_layout) as a dict."""
+        out = _lib.FlGsLayout()
+        check(self._L.fl_get_gs_layout(self._ctx, ctypes.byref(out)))
+        return {k: getattr(out, k) for k, _ in _lib.FlGsLayout._fields_}
+
     def run_pass(self, name, dt=0.0, iters=0):
         """Run one unexported pass (copyBorder, makeIncompressible, advectVelocity, ...)."""
         self._push()

@@ -133,6 +133,21 @@ int fl_get_timing(fl_ctx* ctx, fl_timing* out);
 /* maxDiv of every sweep of the last solve (speculative run, before a rollback). */
 int fl_sweep_maxdiv(fl_ctx* ctx, float* out, int cap, int* count);
 
+/* The tiling of the last temporal-blocked Gauss-Seidel solve on `ctx` (no reference
+ * counterpart: the reference's solve is a serial loop, fluid.go:188-234). For tuning and for
+ * the critical-path model of the solve's time. FL_ERR_INVALID_ARG before the first such solve. */
+typedef struct fl_gs_layout {
+  int32_t sweeps_per_tile;  /* TK */
+  int32_t cols_per_block;   /* BLK: columns per staging and publication block */
+  int32_t handoff_lag;      /* publication used: 1 on time, 2 one block late, 3 halfway */
+  int32_t row_tiles;        /* G: tiles along the skewed rows (64 rows each) */
+  int32_t sweep_blocks;     /* KB: tiles along the sweeps */
+  int32_t steps_per_tile;   /* column steps each tile runs */
+  int32_t tiles;            /* non-empty tiles */
+  int32_t workgroups;       /* persistent workgroups launched */
+} fl_gs_layout;
+int fl_get_gs_layout(fl_ctx* ctx, fl_gs_layout* out);
+
 /* walls.go:5-93 */
 int fl_set_solid(fl_ctx* ctx, int i, int j, int solid);   /* SetSolid  walls.go:5  */
 int fl_is_solid(fl_ctx* ctx, int i, int j, int* out);      /* IsSolid   walls.go:51 */

@@ -72,6 +72,24 @@ def test_many_tiles_bit_exact(shape, early, lag):
     g.close()
 
 
+def test_gs_layout_reports_the_tiling():
+    """fl_get_gs_layout: an error before the first temporal-blocked solve, then the tiling that
+    solve used (150x90, 24 sweeps, 8 sweeps x 4 columns, halfway publication)."""
+    from fluid_amd._lib import FluidError
+    g = Fluid(1000.0, 150, 90, 0.01)
+    with pytest.raises(FluidError, match="no temporal-blocked solve"):
+        g.gs_layout()
+    g.PressureIters, g.EarlyExit = 24, False
+    g.BlockSweeps, g.BlockCols, g.HandoffLag = 8, 4, 3
+    g.Simulate(0.1)
+    L = g.gs_layout()
+    # rows i' = i + k span 0..151+23 over 64-row tiles; 92 + 63 + 7 column steps rounded up to blocks
+    assert (L["sweeps_per_tile"], L["cols_per_block"], L["handoff_lag"]) == (8, 4, 3)
+    assert (L["row_tiles"], L["sweep_blocks"], L["steps_per_tile"]) == (3, 3, 164)
+    assert 1 <= L["workgroups"] <= L["tiles"] <= 9
+    g.close()
+
+
 @pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
 def test_small_grids_every_shape(shape):
     """Grids of one tile, one strip and a few tiles (the critical-path-bound sizes the tuner

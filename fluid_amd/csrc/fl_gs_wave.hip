@@ -123,8 +123,10 @@ __device__ __forceinline__ const SimDesc& sim_desc(const ManySims& m, int e) { r
 // SNAP: the main run also writes the first sweep's U, V, p to T0, T1, T2, so that an early exit
 // after sweep 0 needs no re-run (k_gs_take_first); a separate instantiation, so the usual kernel
 // carries none of its code or registers.
+// 8-sweep tiles of up to 8 columns per block are held to 128 VGPRs (at least 4 waves per SIMD),
+// so two workgroups share a CU: the batched kernel needs 137 otherwise and fits without spills.
 template <int TK, int BLK, class M, bool SNAP = false>
-__global__ __launch_bounds__(TK * LANES) void k_gs_wave(M m, GsRun R) {
+__global__ __launch_bounds__(TK * LANES, (TK == 8 && BLK <= 8) ? 4 : 1) void k_gs_wave(M m, GsRun R) {
   static_assert(LANES % BLK == 0, "progress is published in whole blocks that divide the tile height");
   const int lane = threadIdx.x & (LANES - 1);
   // wave-uniform, and held in a scalar register so every test on it is a scalar branch

@@ -337,6 +337,40 @@ static float time_ms(hipStream_t s, const std::function<int()>& fn, int reps, in
   return ms / reps;
 }
 
+// Time every candidate shape (apply(shape) sets the knobs, then `solve` is timed over 2 runs
+// after a warm-up), then re-time the three fastest twice more, interleaved, and keep the shape
+// with the lowest total: one pass alone ranks shapes within its own noise (the 256-member
+// ensemble's tuner picked 16-sweep tiles at 15.91 ms in one run and 8-sweep tiles at 15.47 ms
+// in another).
+static int tune(const std::vector<TuneShape>& shapes, hipStream_t s, const std::function<int(const TuneShape&)>& apply,
+                const std::function<int()>& solve, TuneShape* best, float* best_ms) {
+  int rc = 0;
+  std::vector<std::pair<float, int>> t;
+  for (size_t i = 0; i < shapes.size(); i++) {
+    if ((rc = apply(shapes[i]))) return rc;
+    const float ms = time_ms(s, solve, 2, &rc);
+    if (rc) return rc;
+    t.push_back({ms, (int)i});
+  }
+  if (t.empty()) return FL_OK;
+  std::sort(t.begin(), t.end());
+  const size_t top = std::min<size_t>(3, t.size());
+  std::vector<float> total(top);
+  for (size_t k = 0; k < top; k++) total[k] = t[k].first;
+  for (int pass = 0; pass < 2; pass++)
+    for (size_t k = 0; k < top; k++) {
+      if ((rc = apply(shapes[t[k].second]))) return rc;
+      total[k] += time_ms(s, solve, 2, &rc);
+      if (rc) return rc;
+    }
+  size_t kb = 0;
+  for (size_t k = 1; k < top; k++)
+    if (total[k] < total[kb]) kb = k;
+  *best = shapes[t[kb].second];
+  *best_ms = total[kb] / 3;
+  return FL_OK;
+}
+
 }  // namespace fl
 
 using namespace fl;
@@ -512,21 +546,17 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms) {
   for (Ctx* c : B->members) tiles8 += tiles_at(c->NX, N, 8);
   TuneShape best{};
   float best_ms = 1e30f;
-  int rc = 0;
   const fl_params saved = B->prm;
-  for (const TuneShape& sh : tune_shapes(tiles8, device_cus(B->dev))) {
-    B->prm.block_sweeps = sh.tk;
-    B->prm.block_cols = sh.blk;
-    B->prm.handoff_lag = sh.lag;
-    if ((rc = prepare_batch(B, dt))) break;
-    // trial solves write only scratch roles (U2, V2, T2), so the members' state is unchanged
-    float t = time_ms(B->stream, [&]() { return batch_gs_solve(B, N, -1, B_T2); }, 2, &rc);
-    if (rc) break;
-    if (t < best_ms) {
-      best_ms = t;
-      best = sh;
-    }
-  }
+  // trial solves write only scratch roles (U2, V2, T2), so the members' state is unchanged
+  int rc = tune(
+      tune_shapes(tiles8, device_cus(B->dev)), B->stream,
+      [&](const TuneShape& sh) {
+        B->prm.block_sweeps = sh.tk;
+        B->prm.block_cols = sh.blk;
+        B->prm.handoff_lag = sh.lag;
+        return prepare_batch(B, dt);
+      },
+      [&]() { return batch_gs_solve(B, N, -1, B_T2); }, &best, &best_ms);
   B->prm = saved;
   if (rc) return rc;
   B->prm.block_sweeps = best.tk;
@@ -552,26 +582,23 @@ int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms) {
   float best_ms = 1e30f;
   const fl_params saved = c->prm;
   c->prm.solver = 2;
-  for (const TuneShape& sh : tune_shapes(tiles_at(c->NX, N, 8), device_cus(c->dev))) {
-    c->prm.block_sweeps = sh.tk;
-    c->prm.block_cols = sh.blk;
-    c->prm.handoff_lag = sh.lag;
-    // the solve swaps U/U2, V/V2, p/T3 towards its outputs; swapping back restores the state
-    float t = time_ms(c->stream,
-                      [&]() {
-                        int r = pressure_solve(c, N, dt);
-                        std::swap(c->U, c->U2);
-                        std::swap(c->V, c->V2);
-                        std::swap(c->p, c->T3);
-                        return r;
-                      },
-                      2, &rc);
-    if (rc) break;
-    if (t < best_ms) {
-      best_ms = t;
-      best = sh;
-    }
-  }
+  rc = tune(
+      tune_shapes(tiles_at(c->NX, N, 8), device_cus(c->dev)), c->stream,
+      [&](const TuneShape& sh) {
+        c->prm.block_sweeps = sh.tk;
+        c->prm.block_cols = sh.blk;
+        c->prm.handoff_lag = sh.lag;
+        return FL_OK;
+      },
+      [&]() {
+        // the solve swaps U/U2, V/V2, p/T3 towards its outputs; swapping back restores the state
+        int r = pressure_solve(c, N, dt);
+        std::swap(c->U, c->U2);
+        std::swap(c->V, c->V2);
+        std::swap(c->p, c->T3);
+        return r;
+      },
+      &best, &best_ms);
   c->prm = saved;
   if (rc) return rc;
   c->prm.block_sweeps = best.tk;

@@ -423,11 +423,13 @@ __global__ __launch_bounds__(TK * LANES, (TK == 8 && BLK <= 8) ? 4 : 1) void k_g
         pn = pnew;
         // ---- the reference cell update (fluid.go:196-229)
         const uint32_t f = PACK ? (fcur[u / 4] >> (8 * (u % 4))) & 0xFFu : fcur[u];
-        if (__builtin_expect(__all((f & 0x3Fu) == 0x3Fu), 1)) {
+        const bool plain = __all((f & 0x3Fu) == 0x3Fu);
+        if (std::is_same<M, OneSim>::value ? __builtin_expect(plain, 1) : plain) {
           // Every lane holds a fluid interior cell whose four neighbours are fluid: s = 4 and
           // all weights are 1, so (-div)/4 is the exact product with 0.25 and the face updates
-          // are plain adds -- the same float results with a third of the instructions. Laid out
-          // as the fall-through path (1% per step on the 4096^2 cavity and the 256^2 jet).
+          // are plain adds -- the same float results with a third of the instructions. For a
+          // single simulation laid out as the fall-through path (1% per step on the 4096^2
+          // cavity and the 256^2 jet); the batched 8x8 kernel would spill at 128 VGPRs with it.
           const float div = Ur - Ul + Vt - Vb;
           const float ad = fabsf(div);
           if (ad > md) md = ad;

@@ -210,8 +210,8 @@ def bench_ensemble(args):
 def run_ensemble(args):
     """The batched ensemble: 256 members of seeded sizes 64..1024, presets, wall strokes,
     Gaussian impulses every 10 steps, 60 sweeps, SL, stepped by one FluidBatch (one launch
-    per pass, one persistent pressure launch, graph replay, the library's default tile shape, or
-    fl_batch_autotune's with --autotune), against
+    per pass, one persistent pressure launch, graph replay, tiles chosen by fl_batch_autotune, or
+    the library's default tile shape with --no-autotune), against
     the same members stepped as 256 separate Fluids (one set of launches per grid)."""
     from fluid_amd import Fluid, FluidBatch
     from fluid_amd import configs as C
@@ -226,7 +226,7 @@ def run_ensemble(args):
     b.update(C.DT)
     b.sync()
     solve_ms = None
-    if getattr(args, "autotune", False):
+    if not getattr(args, "no_autotune", False):
         tk, solve_ms = b.autotune(C.DT)
         print("bench: autotuned tile depth %d (solve %.2f ms)" % (tk, solve_ms), file=sys.stderr, flush=True)
     else:
@@ -287,7 +287,7 @@ def run_ensemble(args):
                                "4 wall strokes each, Gaussian impulses every 10 steps, %d GS sweeps, SL, dt 0.1"
                                % (len(specs), args.iters),
                    "members": len(specs), "cells": cells, "pressure_sweeps": args.iters, "block_sweeps": tk},
-        # the batched solve's own time is known only from the tuner's timing (--autotune); without
+        # the batched solve's own time is known only from the tuner's timing; without
         # it the bound is stated for the whole batched step
         "roofline": ({"bound": "hbm", "achieved": b_alg_solve / (solve_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": b_alg_solve / (solve_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
@@ -404,8 +404,8 @@ def main():
     ap.add_argument("--confinement", type=float, default=0.01,
                     help="differentiable rollout: vorticity confinement (0.3 diverges on the 512^2 jet)")
     ap.add_argument("--no-per-grid", action="store_true")
-    ap.add_argument("--autotune", action="store_true",
-                    help="ensemble: let fl_batch_autotune pick the tile shape instead of the library's default")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="ensemble: step with the library's default tile shape instead of fl_batch_autotune's")
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--solver", type=int, default=0)

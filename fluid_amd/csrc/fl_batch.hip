@@ -339,9 +339,8 @@ static float time_ms(hipStream_t s, const std::function<int()>& fn, int reps, in
 
 // Time every candidate shape (apply(shape) sets the knobs, then `solve` is timed over 2 runs
 // after a warm-up), then re-time the three fastest twice more, interleaved, and keep the shape
-// with the lowest total: one pass alone ranks shapes within its own noise (the 256-member
-// ensemble's tuner picked 16-sweep tiles at 15.91 ms in one run and 8-sweep tiles at 15.47 ms
-// in another).
+// with the lowest total: shapes whose solves differ by less than the run-to-run spread of one
+// timing (a few percent on large batches) are otherwise ranked by chance.
 static int tune(const std::vector<TuneShape>& shapes, hipStream_t s, const std::function<int(const TuneShape&)>& apply,
                 const std::function<int()>& solve, TuneShape* best, float* best_ms) {
   int rc = 0;

@@ -657,8 +657,8 @@ static bool shape_ok(int tk, int blk) {
 // Sweeps per tile: the knob if set; otherwise 4 for solves of at most 8 sweeps (Simulate's own
 // count: one wave per SIMD, 300x251 frame 0.466 -> 0.447 ms), else 8: twice the tiles of 16 and
 // a shorter pipeline fill, and with its flags packed two workgroups per CU. Measured against 16
-// on the 4096^2 cavity (11.8 against 12.4 ms/step, each with its best publication) and equal on
-// the 256-member ensemble.
+// on the 4096^2 cavity (11.8 against 12.4 ms/step, each with its best publication) and on the
+// 256-member ensemble (17.0 against 17.6).
 int pick_tk(const fl_params& P, int N) {
   if (P.block_sweeps == 2 || P.block_sweeps == 4 || P.block_sweeps == 8 || P.block_sweeps == 16)
     return P.block_sweeps;

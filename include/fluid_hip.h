@@ -68,12 +68,13 @@ typedef struct fl_params {
   int32_t pressure_iters;    /* sweeps per Simulate, default 8 */
   int32_t early_exit;        /* 1: stop after the first sweep with maxDiv < 1e-5 */
   int32_t solver;            /* 0 auto, 1 hyperplane kernel, 2 temporal-blocked kernel */
-  int32_t block_sweeps;      /* sweeps per temporal block: 0 auto, 2, 4, 8 or 16 */
+  int32_t block_sweeps;      /* sweeps per temporal block: 0 auto (8; 4 for at most 8 sweeps),
+                                2, 4, 8 or 16 */
   int32_t precise_advection; /* 1: back-trace in cell units (no reference counterpart), default 0 */
   int32_t handoff_lag;       /* temporal-blocked solve: 0 auto, 1 publish each block's handoff rows
                                 at the next block's start, 2 one block later (fewer waits, longer
-                                pipeline fill; auto picks 2 from two rounds of tiles up), 3 halfway
-                                through the next block */
+                                pipeline fill), 3 halfway through the next block; auto picks 3
+                                from two rounds of tiles up, else 1 */
   int32_t block_cols;        /* temporal-blocked solve: columns per staging and publish block,
                                 0 auto (8), 4, 8 or 16 */
 } fl_params;
@@ -233,9 +234,10 @@ int fl_scene_update(fl_ctx* ctx, float dt);
 int fl_scene_draw_line(fl_ctx* ctx, int x0, int y0, int x1, int y1, int solid); /* drawLine main.go:809 */
 int fl_scene_smoke_brush(fl_ctx* ctx, int ci, int cj, int radius, float strength); /* main.go:453 */
 
-/* Gauss-Seidel tile depth autotuner for one context: times the candidate depths of
- * the temporal-blocked solve on the current state (which it leaves unchanged) and
- * keeps the fastest in params.block_sweeps. The reference has no counterpart (its
+/* Gauss-Seidel tile-shape autotuner for one context: times the candidate shapes (sweeps per
+ * tile, columns per block, handoff publication) of the temporal-blocked solve on the current
+ * state (which it leaves unchanged), re-times the three fastest, and keeps the best in
+ * params.block_sweeps, block_cols and handoff_lag. The reference has no counterpart (its
  * solve is a serial loop, fluid.go:188-234). */
 int fl_autotune(fl_ctx* ctx, float dt, int* block_sweeps, float* ms);
 

@@ -639,12 +639,18 @@ static TileGeom geom(int NX, int NY, int N, int tk, int blk) {
   t.rows = (steps + blk - 1) / blk * blk;
   return t;
 }
-static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, int NX, int N, int tk,
+// Tiles are handed out in the order their producers let them start: a tile can start about
+// 64 + 2 BLK columns after the tile one row-tile up and TK + 2 BLK after the tile one sweep block
+// back (the consumer stages a block ahead and rows are published in whole blocks), so the key
+// is that estimate of its start. Producers always get smaller keys (a topological order, which
+// the persistent kernel needs: a workgroup never waits on a tile no workgroup has taken).
+static void append_tiles(std::vector<std::pair<long long, int2>>& out, int sim, int NX, int N, int tk, int blk,
                          const TileGeom& t) {
   for (int g = 0; g < t.G; g++)
     for (int kb = 0; kb < t.KB; kb++)
       if (!tile_empty_h(g, kb, N, tk, NX)) {
-        long long key = ((long long)(LANES * g + kb * tk) << 32) | ((long long)sim << 12) | kb;
+        const long long start = (long long)(LANES + 2 * blk) * g + (long long)(tk + 2 * blk) * kb;
+        long long key = (start << 32) | ((long long)sim << 12) | kb;
         out.push_back({key, make_int2(sim, (g << 16) | kb)});
       }
 }
@@ -731,7 +737,7 @@ static int layout(Ctx* c, int N) {
   W.KB = t.KB;
   W.rows = t.rows;
   std::vector<std::pair<long long, int2>> order;
-  append_tiles(order, 0, c->NX, N, tk, t);
+  append_tiles(order, 0, c->NX, N, tk, blk, t);
   std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
   std::vector<int2> tiles;
   for (auto& o : order) tiles.push_back(o.second);
@@ -854,7 +860,7 @@ int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs) {
   std::vector<TileGeom> tg(descs.size());
   for (size_t e = 0; e < descs.size(); e++) {
     tg[e] = geom(descs[e].g.NX, descs[e].g.NY, N, tk, blk);
-    append_tiles(order, (int)e, descs[e].g.NX, N, tk, tg[e]);
+    append_tiles(order, (int)e, descs[e].g.NX, N, tk, blk, tg[e]);
     const size_t nt = (size_t)tg[e].G * tg[e].KB;
     kelems += nt * tg[e].rows * 3 * (LANES + 1);
     gelems += nt * tg[e].rows * 3 * tk;

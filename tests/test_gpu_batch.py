@@ -229,7 +229,8 @@ def test_autotune_keeps_state():
         s.update(C.DT)
     before = [m.state() for m in b.members]
     tk, ms = b.autotune(C.DT)
-    assert tk in (8, 16) and ms > 0
+    # one of the candidate shapes (2-sweep tiles are tried only on grids this small)
+    assert tk in (2, 4, 8, 16) and b.BlockCols in (4, 8, 16) and b.HandoffLag in (1, 2, 3) and ms > 0
     for m, st in zip(b.members, before):
         for k, v in st.items():
             np.testing.assert_array_equal(m.get(k).view(np.uint32), v.view(np.uint32))
@@ -244,7 +245,7 @@ def test_autotune_keeps_state():
     f = solo[0]
     st = f.state()
     tk1, ms1 = f.autotune(C.DT)
-    assert tk1 in (8, 16) and ms1 > 0
+    assert tk1 in (2, 4, 8, 16) and f.BlockCols in (4, 8, 16) and f.HandoffLag in (1, 2, 3) and ms1 > 0
     for k, v in st.items():
         np.testing.assert_array_equal(f.get(k).view(np.uint32), v.view(np.uint32))
     b.close()

@@ -313,10 +313,19 @@ static int mark(Ctx* c, int k) {
 static int simulate_passes(Ctx* c, float dt) {
   int rc;
   const fl_params& P = c->prm;
-  if ((rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;                      // fill(p,0)
+  // fill(p,0): a temporal-blocked solve starts from zero p without reading it and writes every
+  // p cell (viscosity, in between, does not touch p), so the fill is only launched for the other
+  // solves, or when there is no solve
+  const bool mg = P.use_multigrid && P.multigrid_levels > 1;
+  const int N = P.pressure_iters;
+  const bool p_by_solve = N > 0 && !mg && (P.solver == 2 || (P.solver == 0 && gs_wave_supported(c, N)));
+  if (!p_by_solve && (rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;         // fill(p,0)
   if (P.viscosity_diffusion > 0 && (rc = launch_viscosity(c, dt))) return rc;          // applyViscosity
   if ((rc = mark(c, 1))) return rc;
-  if ((rc = make_incompressible(c, P.pressure_iters, dt, false))) return rc;           // makeIncompressible
+  c->gs_p_zero = p_by_solve;
+  rc = make_incompressible(c, P.pressure_iters, dt, false);                              // makeIncompressible
+  c->gs_p_zero = false;
+  if (rc) return rc;
   if ((rc = mark(c, 2))) return rc;
   const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
   if (conf && (rc = launch_curl(c, c->T0))) return rc;

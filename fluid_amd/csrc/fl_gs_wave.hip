@@ -569,6 +569,29 @@ __global__ void k_gs_init(int32_t* prog, int nprog, uint32_t* maxdiv, int nmd, i
   }
 }
 
+// k_gs_ring and k_gs_init of a single simulation in one launch (both only precede the solve).
+__global__ void k_gs_prep(OneSim m, int rPin, int rPout, int32_t* prog, int nprog, uint32_t* maxdiv, int nmd,
+                          int32_t* ctrl, int32_t* ticket) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const SimDesc& D = m.d;
+  const int NX = D.g.NX, NY = D.g.NY;
+  if (t < NX) {
+    ring_copy(D, t, 0, rPin, rPout);
+    ring_copy(D, t, NY - 1, rPin, rPout);
+  }
+  if (t < NY) {
+    ring_copy(D, 0, t, rPin, rPout);
+    ring_copy(D, NX - 1, t, rPin, rPout);
+  }
+  const int stride = blockDim.x * gridDim.x;
+  for (int i = t; i < nprog; i += stride) prog[i] = 0;
+  for (int i = t; i < nmd; i += stride) maxdiv[i] = 0u;
+  if (t == 0) {
+    ctrl[0] = ctrl[1] = 0;
+    ticket[0] = ticket[1] = ticket[2] = 0;
+  }
+}
+
 // Early exit of solveSingleGrid (fluid.go:175-177), one wave per simulation: the first
 // sweep whose max |div| is below 1e-5 ends the solve; if that is not the last sweep,
 // request a re-run of k*+1 sweeps (and flag the launch, rerun[0], for the re-run kernel),
@@ -767,9 +790,11 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   OneSim m{d};
   const int nprog = W.G * W.KB;
   const int mr = std::max(c->NX, c->NY);
-  hipLaunchKernelGGL(k_gs_ring<OneSim>, dim3((mr + 255) / 256), dim3(256), 0, c->stream, m, (int)B_P, (int)B_T3);
-  hipLaunchKernelGGL(k_gs_init, dim3(4), dim3(256), 0, c->stream, W.d_prog, 2 * nprog, c->d_maxdiv, 2 * c->nmax,
-                     c->d_ctrl, 1, W.d_ticket);
+  // Simulate's fill(p, 0) left to this solve: the initial p is zero and is not read
+  const int rPin = c->gs_p_zero ? -1 : (int)B_P;
+  c->gs_p_zero = false;
+  hipLaunchKernelGGL(k_gs_prep, dim3(std::max(4, (mr + 255) / 256)), dim3(256), 0, c->stream, m, rPin, (int)B_T3,
+                     W.d_prog, 2 * nprog, c->d_maxdiv, 2 * c->nmax, c->d_ctrl, W.d_ticket);
   FL_HIP(hipGetLastError());
   GsRun R{};
   R.tiles = (const int2*)W.d_tile;
@@ -777,7 +802,7 @@ int gs_wave_solve(Ctx* c, int N, float cp) {
   R.K = N;
   R.omega = c->d_omega;
   R.damping = c->prm.pressure_damping;
-  R.rPin = B_P;  // then p and T3 trade places
+  R.rPin = rPin;  // then p and T3 trade places
   R.rPout = B_T3;
   R.defer = defer_handoff(c->prm, W.ntiles, W.grid);
   W.defer = R.defer;

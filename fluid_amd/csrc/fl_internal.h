@@ -128,6 +128,9 @@ struct Ctx {
   float density = 0, h = 0;
   fl_params prm{};
   Grid grid{};
+  // Simulate's fill(p, 0) is left to its temporal-blocked solve, which then starts from p = 0
+  // without reading p (set by the step around its solve only)
+  bool gs_p_zero = false;
 
   // Device state: the 8 reference buffers plus ping-pong partners and scratch.
   float *U = nullptr, *V = nullptr, *nU = nullptr, *nV = nullptr, *p = nullptr, *S = nullptr, *M = nullptr,
@@ -349,6 +352,7 @@ int batch_apply_edits(Batch* B, const uint32_t* keys, const float* vals, const i
                       int nseg);
 // batched pressure solve (fl_gs_wave.hip): sets the GS fields of `descs`
 int pick_tk(const fl_params& P, int N);
+bool gs_wave_supported(Ctx* c, int numIters);
 int pick_blk(const fl_params& P);
 int batch_gs_layout(Batch* B, int N, std::vector<SimDesc>& descs);
 int batch_gs_solve(Batch* B, int N, int rPin, int rPout);  // p from rPin (< 0: zero), to rPout

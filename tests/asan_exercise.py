@@ -105,6 +105,7 @@ def main():
             step(f, 1)
     f.BlockSweeps = f.BlockCols = f.HandoffLag = 0
     f.autotune(C.DT)
+    f.gs_layout()
     f.close()
 
     # staged host copies (fields above the 2 MB staging threshold), both ways

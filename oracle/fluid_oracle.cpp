@@ -55,6 +55,9 @@ namespace orc {
 //  21 GetAdaptiveTimeStep: speed sqrt(U^2+V^2)             22 drawLine: e2 > dy instead of e2 >= dy
 //  23 cavity preset: source row at NY-3                    24 applySources: solid cells not skipped
 //  25 Karman preset: disc centred at NX/2                  26 applyWallSettings: columns before rows
+//  27 multigrid adjoint: residual's p term with the opposite sign
+//  28 multigrid adjoint: coarse sweep's own (1 - relax) factor dropped
+//  29 multigrid adjoint: correction's cp * lambda_p term dropped
 #ifdef ORC_PLANTABLE
 static int g_plant = 0;
 #define PLANT(k) (g_plant == (k))

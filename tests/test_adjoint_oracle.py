@@ -82,3 +82,47 @@ def test_adjoint_with_viscosity(bfecc, disc):
     o.ViscosityDiffusion = 0.15
     assert max(fd_errors(o, 3, 1e-7)) < 1e-5
     assert max(fd_errors(o, 3, 1e-7, seed=3)) < 1e-5
+
+
+def mg_scene(iters=4, disc=False, bfecc=False, jet_u=1.0, turb=0.02, prec=64):
+    """The multigrid V-cycle (fluid.go:560-599) at density dt/h, so cp = density * h / dt = 1.
+    The cycle subtracts lap(p) from the divergence while p accumulates cp * q (fluid.go:601-632,
+    1123-1149); at the scenes' density 1000 (cp = 1000) that correction overshoots and the
+    divergence grows about tenfold per cycle, the flow saturates the sampler's clamps and the
+    loss stops depending on F through the solve (the transposed passes then see no adjoint)."""
+    o = OracleFluid(C.DT / C.H, 20, 16, C.H, prec=prec)
+    C.jet_scene(lambda *a: o, 20, 16, iters, jet_u=jet_u, jet_rows=4)
+    if disc:
+        o.SetCircularObstacle(10, 8, 2)
+    o.UseBFECC, o.Confinement, o.TurbulenceStrength, o.EarlyExit = bfecc, 0.0, turb, False
+    o.UseMultigrid, o.MultigridLevels = True, 2
+    return o
+
+
+@pytest.mark.parametrize("bfecc", [False, True], ids=["sl", "bfecc"])
+@pytest.mark.parametrize("disc", [False, True], ids=["box", "disc"])
+@pytest.mark.parametrize("steps", [1, 3])
+def test_adjoint_through_multigrid(bfecc, disc, steps):
+    """The V-cycle's smoothing sweeps with p's adjoint, correction, prolongation, 40 coarse
+    sweeps, restriction and residual, each transposed (fluid_adjoint.inc mg_adj)."""
+    o = mg_scene(disc=disc, bfecc=bfecc)
+    assert max(fd_errors(o, steps, 1e-6)) < 1e-5
+    assert o.last_sweeps() == 24  # four full cycles, no exit
+
+
+def test_adjoint_through_multigrid_exit():
+    """A flow with divergence below 1e-5: the exit test after the first pre-smoothing stops the
+    V-cycle (3 sweeps), and only those sweeps are transposed."""
+    o = mg_scene(iters=3, jet_u=1e-6, turb=0.0)
+    assert max(fd_errors(o, 1, 1e-8, fscale=1e-7)) < 1e-5
+    assert o.last_sweeps() == 3
+
+
+@pytest.mark.parametrize("plant", [27, 28, 29])
+def test_multigrid_adjoint_pin_catches_planted_mistakes(plant):
+    """27: the residual's p term with the opposite sign; 28: the coarse sweep's own (1 - relax)
+    factor dropped; 29: the correction's cp * lambda_p term dropped."""
+    from oracle import pyoracle
+    with pyoracle.planted(plant):
+        o = mg_scene(disc=True)
+        assert max(fd_errors(o, 2, 1e-6)) > 1e-3

@@ -27,7 +27,9 @@
 //                          joins it there.
 //   sources and force      the force's gradient is the adjoint entering Simulate; cells set
 //                          by SetVelocity lose theirs.
-// The multigrid solver is not differentiated (FL_ERR_UNSUPPORTED).
+//   multigrid V-cycle      with its exit sweep fixed, linear in (U,V,p): smoothing sweeps,
+//                          correction, prolongation, coarse sweeps, restriction and residual,
+//                          each transposed and run in reverse (adj_multigrid below).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -788,7 +790,8 @@ static int vjp_step(Vjp* J, float dt, float* tp, int* kept) {
   FL_HIP(hipGetLastError());
   if ((rc = launch_fill(c->p, 0.0f, c->N, c->stream))) return rc;
   if (P.viscosity_diffusion > 0.0f && (rc = launch_viscosity(c, dt))) return rc;
-  if ((rc = pressure_solve(c, P.pressure_iters, dt))) return rc;
+  const bool mg = P.use_multigrid && P.multigrid_levels > 1;  // makeIncompressible (fluid.go:148-154)
+  if ((rc = mg ? multigrid_solve(c, P.pressure_iters, dt) : pressure_solve(c, P.pressure_iters, dt))) return rc;
   if (tp && ((rc = dcopy(J, tp + T_U2 * N, c->U)) || (rc = dcopy(J, tp + T_V2 * N, c->V)))) return rc;
   const bool conf = P.confinement != 0.0f, turb = P.turbulence_strength > 0.0f;
   if (conf && ((rc = launch_curl(c, c->T0)) || (rc = launch_conf_turb(c, c->T0, dt, 1, 0)))) return rc;
@@ -850,6 +853,223 @@ static int mirror_omega(Vjp* J, int K) {
   FL_HIP(hipMemcpy(m->d_omega, om.data(), sizeof(float) * K, hipMemcpyHostToDevice));
   m->omega_n = -1;  // the table is not the forward schedule
   J->mir_K = K;
+  return FL_OK;
+}
+
+// ---- the multigrid V-cycle transposed (fluid.go:560-758, 1123-1149) -----------------------
+// With the masks and the exit sweep fixed the cycle is linear in (U, V, p). p enters through
+// the residual, so its adjoint lp is carried back through the smoothing sweeps, whose
+// p += cp q gives q's adjoint a cp * lp term; p leaves the step unused, so lp starts at zero.
+// The sweeps run as hyperplanes t = i + j + 2k in decreasing t: the cells of one hyperplane
+// touch disjoint words (as in the forward schedule of fl_pressure.hip), so each launch is a
+// set of independent transposed cell updates.
+
+// Smoothing sweeps (pressureJacobiIteration, fluid.go:188-234) transposed: cell (i,j) of
+// sweep k in hyperplane t.
+__global__ void k_adj_gs_hp(int NX, int NY, int t, int k0, const uint8_t* __restrict__ flags, float om, float damping,
+                            float cp, float* __restrict__ lU, float* __restrict__ lV, const float* __restrict__ lp) {
+  const int k = k0 + blockIdx.y;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = t - i - 2 * k;
+  if (i > NX - 2 || j < 1 || j > NY - 2) return;
+  const int n = NY, c = i * n + j;
+  const uint32_t f = flags[c];
+  if (!(f & CF_GS)) return;
+  const float sx0 = (f & CF_X0) ? 1.0f : 0.0f, sx1 = (f & CF_X1) ? 1.0f : 0.0f;
+  const float sy0 = (f & CF_Y0) ? 1.0f : 0.0f, sy1 = (f & CF_Y1) ? 1.0f : 0.0f;
+  const float s = sx0 + sx1 + sy0 + sy1;
+  const float alpha = -(om * damping) / s;
+  const float g = alpha * (sx1 * lU[c + n] - sx0 * lU[c] + sy1 * lV[c + 1] - sy0 * lV[c] + cp * lp[c]);
+  lU[c] -= g;
+  lU[c + n] += g;
+  lV[c] -= g;
+  lV[c + 1] += g;
+}
+
+// Three sweeps at one relaxation, transposed.
+static void adj_mg_sweeps(Ctx* c, float om, float cp, float* lU, float* lV, const float* lp) {
+  const int NX = c->NX, NY = c->NY, K = 3, threads = 256;
+  const int bx = (NX - 2 + threads - 1) / threads;
+  for (int t = (NX - 2) + (NY - 2) + 2 * (K - 1); t >= 2; t--) {
+    const int klo = std::max(0, (t - (NX - 2) - (NY - 2) + 1) / 2);
+    const int khi = std::min(K - 1, (t - 2) / 2);
+    if (klo > khi) continue;
+    hipLaunchKernelGGL(k_adj_gs_hp, dim3(bx, khi - klo + 1), dim3(threads), 0, c->stream, NX, NY, t, klo, c->flags,
+                       om, c->prm.pressure_damping, cp, lU, lV, lp);
+  }
+}
+
+__device__ __forceinline__ bool mg_fluid_interior(const Grid& g, const float* __restrict__ S, int i, int j) {
+  return i >= 1 && i <= g.NX - 2 && j >= 1 && j <= g.NY - 2 && S[i * g.NY + j] != 0.0f;
+}
+
+// applyPressureCorrection (fluid.go:1123-1149) transposed: the correction's adjoint at each
+// fluid interior cell (p += cp corr; U, V faces -= / += S corr). The adjoints of U, V, p pass
+// through unchanged.
+__global__ void k_adj_mg_apply(Grid g, const float* __restrict__ S, float cp, const float* __restrict__ lU,
+                               const float* __restrict__ lV, const float* __restrict__ lp, float* __restrict__ lcorr) {
+  const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.NX || j >= g.NY) return;
+  const int n = g.NY, c = i * n + j;
+  float v = 0.0f;
+  if (mg_fluid_interior(g, S, i, j))
+    v = cp * lp[c] - S[c - n] * lU[c] + S[c + n] * lU[c + n] - S[c - 1] * lV[c] + S[c + 1] * lV[c + 1];
+  lcorr[c] = v;
+}
+
+// prolongateCorrection (fluid.go:719-756) transposed: coarse value (I,J) gathers the fine
+// values it was interpolated into, i in {2I-1, 2I, 2I+1} with weight 1/2, 1, 1/2 (likewise j),
+// for fine cells with i/2 < cNX-1 and j/2 < cNY-1.
+__global__ void k_adj_mg_prolong(int NX, int NY, int cNX, int cNY, const float* __restrict__ lcorr,
+                                 float* __restrict__ lP) {
+  const int J = blockIdx.x * BX + threadIdx.x, I = blockIdx.y * BY + threadIdx.y;
+  if (I >= cNX || J >= cNY) return;
+  float acc = 0.0f;
+  for (int di = -1; di <= 1; di++) {
+    const int i = 2 * I + di;
+    if (i < 0 || i >= NX || i / 2 >= cNX - 1) continue;
+    const float wi = di == 0 ? 1.0f : 0.5f;
+    for (int dj = -1; dj <= 1; dj++) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j >= NY || j / 2 >= cNY - 1) continue;
+      acc += (wi * (dj == 0 ? 1.0f : 0.5f)) * lcorr[i * NY + j];
+    }
+  }
+  lP[I * cNY + J] = acc;
+}
+
+// solveCoarseGrid (fluid.go:667-716) transposed. Its update of coarse cell c at sweep k,
+// P_c <- P_c + relax/s_c (sum_d CS_d P_d - s_c P_c - rhs_c), read in reverse as a gather:
+// the adjoint of P_c after sweep k is (1 - relax) times that after sweep k+1 (its own next
+// update), plus relax/s_d times the adjoint after the update of each neighbour d that reads it:
+// (i+1,j) and (i,j+1) at sweep k, (i-1,j) and (i,j-1) at sweep k+1. Every one of these lies
+// on hyperplane t+1 or t+2, and lP holds each cell's adjoint in place (its neighbours' words
+// are on hyperplanes of the other parity), so a decreasing-t schedule needs no atomics. The
+// rhs adjoint collects -relax/s_c times each adjoint after an update.
+__device__ __forceinline__ float mg_cs(const float* __restrict__ S, int NX, int NY, int a, int b) {
+  return (2 * a < NX && 2 * b < NY) ? S[2 * a * NY + 2 * b] : 0.0f;  // fluid.go:672-684
+}
+__device__ __forceinline__ float mg_cs_sum(const float* __restrict__ S, int NX, int NY, int a, int b) {
+  return mg_cs(S, NX, NY, a - 1, b) + mg_cs(S, NX, NY, a + 1, b) + mg_cs(S, NX, NY, a, b - 1) +
+         mg_cs(S, NX, NY, a, b + 1);
+}
+__device__ __forceinline__ bool mg_cupd(const float* __restrict__ S, int NX, int NY, int cNX, int cNY, int a,
+                                        int b) {
+  return a >= 1 && a <= cNX - 2 && b >= 1 && b <= cNY - 2 && mg_cs(S, NX, NY, a, b) != 0.0f &&
+         mg_cs_sum(S, NX, NY, a, b) != 0.0f;
+}
+__global__ void k_adj_mg_coarse(int NX, int NY, int cNX, int cNY, int t, int k0, int K, float relax,
+                                const float* __restrict__ S, float* __restrict__ lP, float* __restrict__ lrhs) {
+  const int k = k0 + blockIdx.y;
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = t - i - 2 * k;
+  if (i > cNX - 2 || j < 1 || j > cNY - 2) return;
+  if (!mg_cupd(S, NX, NY, cNX, cNY, i, j)) return;
+  const int c = i * cNY + j;
+  float L = lP[c];
+  if (k < K - 1) L = L - relax * L;
+  float acc = 0.0f;
+  if (mg_cupd(S, NX, NY, cNX, cNY, i + 1, j)) acc += lP[c + cNY] / mg_cs_sum(S, NX, NY, i + 1, j);
+  if (mg_cupd(S, NX, NY, cNX, cNY, i, j + 1)) acc += lP[c + 1] / mg_cs_sum(S, NX, NY, i, j + 1);
+  if (k < K - 1) {
+    if (mg_cupd(S, NX, NY, cNX, cNY, i - 1, j)) acc += lP[c - cNY] / mg_cs_sum(S, NX, NY, i - 1, j);
+    if (mg_cupd(S, NX, NY, cNX, cNY, i, j - 1)) acc += lP[c - 1] / mg_cs_sum(S, NX, NY, i, j - 1);
+  }
+  L += relax * acc;
+  lP[c] = L;
+  lrhs[c] -= (relax / mg_cs_sum(S, NX, NY, i, j)) * L;
+}
+
+// restrictResidual (fluid.go:635-664) transposed: fine cell (i,j) gathers the coarse values
+// whose 3x3 full-weighting stencil covers it, weights (1/2 or 1/4 per axis: 1/4, 1/8, 1/16).
+// Only fluid interior cells carry a residual; the others get zero.
+__global__ void k_adj_mg_restrict(Grid g, int cNX, int cNY, const float* __restrict__ S, const float* __restrict__ lrhs,
+                                  float* __restrict__ lr) {
+  const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.NX || j >= g.NY) return;
+  float acc = 0.0f;
+  if (mg_fluid_interior(g, S, i, j)) {
+    for (int di = -1; di <= 1; di++) {
+      const int v = i + di;
+      if (v & 1) continue;
+      const int I = v / 2;
+      if (I < 1 || I > cNX - 2 || 2 * I >= g.NX - 1) continue;
+      const float wi = di == 0 ? 0.5f : 0.25f;
+      for (int dj = -1; dj <= 1; dj++) {
+        const int u = j + dj;
+        if (u & 1) continue;
+        const int Jc = u / 2;
+        if (Jc < 1 || Jc > cNY - 2 || 2 * Jc >= g.NY - 1) continue;
+        acc += (wi * (dj == 0 ? 0.5f : 0.25f)) * lrhs[I * cNY + Jc];
+      }
+    }
+  }
+  lr[i * g.NY + j] = acc;
+}
+
+// computePressureResidual (fluid.go:601-632) transposed, r = -div - lap(p) on fluid interior
+// cells, gathered per face and cell: U face (i,j) gets r(i,j) - r(i-1,j), V face (i,j) gets
+// r(i,j) - r(i,j-1), and p at (i,j) gets s r(i,j) - S(i,j) (sum of its neighbours' r).
+__global__ void k_adj_mg_residual(Grid g, const float* __restrict__ S, const float* __restrict__ lr,
+                                  float* __restrict__ lU, float* __restrict__ lV, float* __restrict__ lp) {
+  const int j = blockIdx.x * BX + threadIdx.x, i = blockIdx.y * BY + threadIdx.y;
+  if (i >= g.NX || j >= g.NY) return;
+  const int n = g.NY, c = i * n + j;
+  const bool self = mg_fluid_interior(g, S, i, j);
+  const float r = self ? lr[c] : 0.0f;
+  float du = r, dv = r, nb = 0.0f;
+  if (mg_fluid_interior(g, S, i - 1, j)) {
+    du -= lr[c - n];
+    nb += lr[c - n];
+  }
+  if (mg_fluid_interior(g, S, i, j - 1)) {
+    dv -= lr[c - 1];
+    nb += lr[c - 1];
+  }
+  if (mg_fluid_interior(g, S, i + 1, j)) nb += lr[c + n];
+  if (mg_fluid_interior(g, S, i, j + 1)) nb += lr[c + 1];
+  lU[c] += du;
+  lV[c] += dv;
+  float dp = -S[c] * nb;
+  if (self) dp += (S[c - n] + S[c + n] + S[c - 1] + S[c + 1]) * r;
+  lp[c] += dp;
+}
+
+// solveMultigridVCycle (fluid.go:560-599) transposed on L_OU/L_OV: `sweeps` fine sweeps ran,
+// 6 per full cycle and 3 more when the exit test after a pre-smoothing stopped the solve.
+// Work fields: L_CURL (lp), L_LCD (the correction's and the residual's adjoints), L_GX and
+// L_GY (coarse P and rhs adjoints); all are free once confinement's adjoint is done.
+static int adj_multigrid(Vjp* J, int sweeps, float dt) {
+  Ctx* c = J->c;
+  const int NX = c->NX, NY = c->NY, cNX = (NX + 1) / 2, cNY = (NY + 1) / 2, KC = 40;
+  const size_t N = J->N, cN = (size_t)cNX * cNY;
+  const Grid g = c->desc().g;
+  const dim3 gr = grid_of(c), bl(BX, BY), cgr((cNY + BX - 1) / BX, (cNX + BY - 1) / BY);
+  hipStream_t st = c->stream;
+  float** L = J->lam;
+  float *lU = L[L_OU], *lV = L[L_OV], *lp = L[L_CURL], *lf = L[L_LCD], *lP = L[L_GX], *lrhs = L[L_GY];
+  const float cp = c->density * c->h / dt, relax = 1.6f;
+  int rc;
+  if ((rc = launch_fill(lp, 0.0f, N, st))) return rc;
+  if (sweeps % 6 == 3) adj_mg_sweeps(c, 1.5f, cp, lU, lV, lp);
+  for (int it = 0; it < sweeps / 6; it++) {
+    adj_mg_sweeps(c, 1.2f, cp, lU, lV, lp);
+    hipLaunchKernelGGL(k_adj_mg_apply, gr, bl, 0, st, g, c->S, cp, lU, lV, lp, lf);
+    hipLaunchKernelGGL(k_adj_mg_prolong, cgr, bl, 0, st, NX, NY, cNX, cNY, lf, lP);
+    if ((rc = launch_fill(lrhs, 0.0f, cN, st))) return rc;
+    const int threads = 256, bx = (cNX - 2 + threads - 1) / threads;
+    for (int t = (cNX - 2) + (cNY - 2) + 2 * (KC - 1); t >= 2; t--) {
+      const int klo = std::max(0, (t - (cNX - 2) - (cNY - 2) + 1) / 2);
+      const int khi = std::min(KC - 1, (t - 2) / 2);
+      if (klo > khi) continue;
+      hipLaunchKernelGGL(k_adj_mg_coarse, dim3(bx, khi - klo + 1), dim3(threads), 0, st, NX, NY, cNX, cNY, t, klo, KC,
+                         relax, c->S, lP, lrhs);
+    }
+    hipLaunchKernelGGL(k_adj_mg_restrict, gr, bl, 0, st, g, cNX, cNY, c->S, lrhs, lf);
+    hipLaunchKernelGGL(k_adj_mg_residual, gr, bl, 0, st, g, c->S, lf, lU, lV, lp);
+    adj_mg_sweeps(c, 1.5f, cp, lU, lV, lp);
+  }
+  FL_HIP(hipGetLastError());
   return FL_OK;
 }
 
@@ -916,7 +1136,9 @@ static int adj_step(Vjp* J, const float* tp, int K, float dt) {
     hipLaunchKernelGGL(k_adj_conf_c, gr, bl, 0, st, g, L[L_LCURL], L[L_OU], L[L_OV]);
   }
   FL_HIP(hipGetLastError());
-  if (K > 0) {
+  if (P.use_multigrid && P.multigrid_levels > 1) {
+    if (K > 0 && (rc = adj_multigrid(J, K, dt))) return rc;
+  } else if (K > 0) {
     Ctx* m = J->mir;
     if ((rc = mirror_omega(J, K))) return rc;
     hipLaunchKernelGGL(k_mirror_in, gr, bl, 0, st, g, c->S, L[L_OU], L[L_OV], m->U, m->V, m->p);
@@ -959,9 +1181,6 @@ static void vjp_free(Vjp* J) {
 }
 
 static int check_supported(Ctx* c) {
-  const fl_params& P = c->prm;
-  if (P.use_multigrid && P.multigrid_levels > 1)
-    return verr(FL_ERR_UNSUPPORTED, "the adjoint does not differentiate the multigrid solver");
   if (c->NX < 6 || c->NY < 6) return verr(FL_ERR_UNSUPPORTED, "the adjoint needs at least 4x4 interior cells");
   return FL_OK;
 }

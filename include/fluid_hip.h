@@ -271,7 +271,7 @@ int fl_batch_autotune(fl_batch* b, float dt, int* block_sweeps, float* ms);
  * checkpointing every `checkpoint_every` steps) and leaves the final state in the context;
  * fl_vjp_backward returns dL/dF by the hand-written adjoint of every pass, recomputing
  * each segment from its checkpoint. Arrays hold NumX*NumY floats (index i*NumY + j).
- * The multigrid solver is not differentiated (FL_ERR_UNSUPPORTED). */
+ * With UseMultigrid the V-cycle (fluid.go:560-758) is differentiated too. */
 typedef struct fl_vjp fl_vjp;
 int fl_vjp_create(fl_ctx* ctx, int steps, int checkpoint_every, fl_vjp** out);
 int fl_vjp_destroy(fl_vjp* v);

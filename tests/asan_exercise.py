@@ -144,6 +144,11 @@ def main():
     r.forward(C.DT, z, z, C.gaussian_blobs(f.NumX, f.NumY))
     r.backward()
     r.close()
+    f.UseMultigrid = True  # the rollout through the V-cycle
+    r = Rollout(f, 3, checkpoint_every=2)
+    r.forward(C.DT, z, z, C.gaussian_blobs(f.NumX, f.NumY))
+    r.backward()
+    r.close()
     f.close()
     print("asan exercise ok", flush=True)
 

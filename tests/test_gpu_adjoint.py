@@ -123,9 +123,45 @@ def test_early_exit_sweeps_follow_the_forward():
     r.close()
 
 
-def test_rejects_multigrid():
-    from fluid_amd import FluidError
-    g = jet(lambda *a: Fluid(*a), 16, 16, 4, False, 0.0)
-    g.UseMultigrid, g.MultigridLevels = True, 2
-    with pytest.raises(FluidError):
-        Rollout(g, 2)
+MG_DENSITY = C.DT / C.H  # cp = density * h / dt = 1
+
+
+@pytest.mark.parametrize("case", ["sl", "bfecc", "bfecc_disc"])
+def test_gradient_through_multigrid_matches_oracle(case):
+    """UseMultigrid: the V-cycle's adjoint (smoothing sweeps carrying p's adjoint, correction,
+    prolongation, 40 coarse sweeps, restriction and residual; fluid.go:560-758, 1123-1149
+    transposed) against the oracle's, at density dt/h. There cp = 1 and the reference's cycle
+    converges; at density 1000 its correction overshoots, the flow saturates the sampler's
+    clamps and the solve's adjoint is zero. With the disc, float32 and FP64 trajectories part
+    at a kink (their adjoints differ by 1e-2), so that case is compared in float32 only."""
+    w, h, iters, steps = 40, 32, 4, 6
+    bfecc, disc = case != "sl", case == "bfecc_disc"
+
+    def make(cls, **kw):
+        return lambda _density, *a: cls(MG_DENSITY, *a, **kw)
+
+    fs = [jet(m, w, h, iters, bfecc, 0.0, disc=disc) for m in (make(Fluid), make(OracleFluid),
+                                                                 make(OracleFluid, prec=64))]
+    for f in fs:
+        f.UseMultigrid, f.MultigridLevels = True, 2
+    g, o32, o64 = fs
+    rng = np.random.default_rng(7)
+    shape = (g.NumX, g.NumY)
+    Fu = (0.02 * rng.standard_normal(shape)).astype(np.float32)
+    Fv = (0.02 * rng.standard_normal(shape)).astype(np.float32)
+    T = C.gaussian_blobs(g.NumX, g.NumY, sigma=4.0)
+    r = Rollout(g, steps, checkpoint_every=4)
+    loss = r.forward(C.DT, Fu, Fv, T)
+    gu, gv = r.backward()
+    r.close()
+    l32, u32, v32 = o32.rollout(steps, C.DT, Fu, Fv, T)
+    assert o32.last_sweeps() == 6 * iters  # full cycles: the transposed path has every pass
+    assert abs(loss - l32) <= 1e-9 * abs(l32)  # same float32 trajectory
+    G, G32 = np.stack([gu, gv]), np.stack([u32, v32])
+    assert np.abs(G32).max() > 0
+    assert rel(G, G32) < 1e-4, rel(G, G32)
+    if disc:
+        return
+    l64, u64, v64 = o64.rollout(steps, C.DT, Fu.astype(np.float64), Fv.astype(np.float64), T.astype(np.float64))
+    G64 = np.stack([u64, v64])
+    assert rel(G, G64) < 1e-3, rel(G, G64)

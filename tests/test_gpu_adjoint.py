@@ -165,3 +165,32 @@ def test_gradient_through_multigrid_matches_oracle(case):
     l64, u64, v64 = o64.rollout(steps, C.DT, Fu.astype(np.float64), Fv.astype(np.float64), T.astype(np.float64))
     G64 = np.stack([u64, v64])
     assert rel(G, G64) < 1e-3, rel(G, G64)
+
+
+def test_gradient_through_multigrid_exit_matches_oracle():
+    """A flow whose divergence stays below 1e-5 (jet speed 1e-6, force 1e-7): each V-cycle
+    stops at the exit test after its first pre-smoothing (3 sweeps, fluid.go:576-578), and
+    the device transposes only those sweeps."""
+    w, h, iters, steps = 40, 32, 4, 4
+    fs = []
+    for m in (lambda _d, *a: Fluid(MG_DENSITY, *a), lambda _d, *a: OracleFluid(MG_DENSITY, *a)):
+        f = C.jet_scene(m, w, h, iters, jet_u=1e-6, jet_rows=8)
+        f.UseBFECC, f.Confinement, f.EarlyExit, f.TurbulenceStrength = False, 0.0, False, 0.0
+        f.UseMultigrid, f.MultigridLevels = True, 2
+        fs.append(f)
+    g, o32 = fs
+    rng = np.random.default_rng(7)
+    shape = (g.NumX, g.NumY)
+    Fu = (1e-7 * rng.standard_normal(shape)).astype(np.float32)
+    Fv = (1e-7 * rng.standard_normal(shape)).astype(np.float32)
+    T = C.gaussian_blobs(g.NumX, g.NumY, sigma=4.0)
+    r = Rollout(g, steps, checkpoint_every=2)
+    loss = r.forward(C.DT, Fu, Fv, T)
+    gu, gv = r.backward()
+    r.close()
+    l32, u32, v32 = o32.rollout(steps, C.DT, Fu, Fv, T)
+    assert o32.last_sweeps() == 3
+    assert abs(loss - l32) <= 1e-9 * abs(l32)
+    G, G32 = np.stack([gu, gv]), np.stack([u32, v32])
+    assert np.abs(G32).max() > 0
+    assert rel(G, G32) < 1e-4, rel(G, G32)

@@ -226,11 +226,16 @@ __global__ void k_mg_restrict(Grid g, int cNX, int cNY, const float* __restrict_
 // (row - 1, l) and (row, l - 1) (step d - 1, this lane and lane l - 1) and before (row + 1, l)
 // and (row, l + 1). Every step runs the reference's expression on the same operands, so the
 // result is the serial loop's to the bit.
-static constexpr int MG_COARSE_ITERS = 40, MG_TR = 32, MG_TC = 64, MG_PITCH = MG_TC + 2;
+// ALL: the interior is one tile of at most TR x 64 cells, and the launch runs all 40 sweeps on
+// it in LDS, one walk after another (each walk reads what the previous one wrote: same wave,
+// LDS in program order), instead of one launch per sweep.
+static constexpr int MG_COARSE_ITERS = 40, MG_TR = 32, MG_TR_ALL = 64, MG_TC = 64, MG_PITCH = MG_TC + 2;
 
+template <int TR, bool ALL>
 __global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int nI, int nJ, int T, int klo,
                                                          const float* __restrict__ cS,
                                                          const float* __restrict__ rhs, float* __restrict__ cP) {
+  constexpr int MG_TR = TR;
   const int k = klo + blockIdx.y, I = blockIdx.x, J = T - 2 * k - I;
   if (J < 0 || J >= nJ) return;
   __shared__ float lp[(MG_TR + 2) * MG_PITCH], ls[(MG_TR + 2) * MG_PITCH], lr[MG_TR * MG_TC];
@@ -271,7 +276,8 @@ __global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int n
   // narrower tile, lane `cols` walks the east halo column. LDS is only read here (the cells a
   // lane will still read are never the ones written), so the walk needs no barrier; results
   // go to LDS for the write-back.
-  if (tid < 64) {
+  if (tid < 64)
+  for (int sweep = 0; sweep < (ALL ? MG_COARSE_ITERS : 1); sweep++) {
   const float relax = 1.6f;
   auto row_of = [&](int x) { return min(max(x, -1), rows) + 1; };  // LDS row of row x, clamped
   const int col = l + 1;                                                // LDS column of this lane
@@ -321,16 +327,23 @@ __global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int n
   }
 }
 
-// All 40 coarse sweeps: one launch per tile diagonal, in order on the stream.
+// All 40 coarse sweeps: one launch when the interior fits one tile of 64 x 64, else one
+// launch per tile diagonal, in order on the stream.
 static int mg_coarse_solve(Ctx* c, int cNX, int cNY) {
   const int W = cNX - 2, H = cNY - 2, K = MG_COARSE_ITERS;
   if (W < 1 || H < 1) return FL_OK;
+  if (W <= MG_TR_ALL && H <= MG_TC) {
+    hipLaunchKernelGGL((k_mg_coarse_tiles<MG_TR_ALL, true>), dim3(1, 1), dim3(256), 0, c->stream, cNX, cNY, 1, 1, 0,
+                       0, c->mg_S, c->mg_R, c->mg_P);
+    FL_HIP(hipGetLastError());
+    return FL_OK;
+  }
   const int nI = (W + MG_TR - 1) / MG_TR, nJ = (H + MG_TC - 1) / MG_TC;
   for (int T = 0; T <= (nI - 1) + (nJ - 1) + 2 * (K - 1); T++) {
     const int klo = std::max(0, (T - (nI - 1) - (nJ - 1) + 1) / 2), khi = std::min(K - 1, T / 2);
     if (klo > khi) continue;  // no tile on this diagonal (a one-tile grid has none on odd T)
-    hipLaunchKernelGGL(k_mg_coarse_tiles, dim3(nI, khi - klo + 1), dim3(256), 0, c->stream, cNX, cNY, nI, nJ, T,
-                       klo, c->mg_S, c->mg_R, c->mg_P);
+    hipLaunchKernelGGL((k_mg_coarse_tiles<MG_TR, false>), dim3(nI, khi - klo + 1), dim3(256), 0, c->stream, cNX, cNY,
+                       nI, nJ, T, klo, c->mg_S, c->mg_R, c->mg_P);
   }
   FL_HIP(hipGetLastError());
   return FL_OK;

@@ -330,7 +330,10 @@ def test_large_transfers_round_trip(w, h):
     f.close()
 
 
-MG_SIZES = [(5, 3), (6, 4), (37, 23), (64, 64), (101, 60)]
+# the coarse solve runs all 40 sweeps in one launch up to a 64 x 64 coarse interior (fine
+# interiors up to 129 x 129), and by tile diagonals beyond (131 x 62 and 62 x 131 are one coarse
+# row or column over)
+MG_SIZES = [(5, 3), (6, 4), (37, 23), (64, 64), (101, 60), (129, 129), (131, 62), (62, 131)]
 
 
 @pytest.mark.parametrize("w,h", MG_SIZES)
@@ -340,7 +343,8 @@ def test_multigrid_bit_exact(w, h, bfecc, scale):
     """UseMultigrid (fluid.go:148-151, 559-758): pre/post smoothing, residual, full-weighting
     restriction, 40 coarse sweeps, bilinear prolongation and the serial correction. The
     reference's V-cycle diverges on the strong random states (NaN after a few steps, at the
-    same cells on both sides); the mild ones stay finite."""
+    same cells on both sides); the mild ones stay finite, but for 0.5% of the U faces of the
+    129 x 129 grid in its third step (on both sides alike)."""
     g, o = pair(1000.0, w, h, 0.01)
     st = random_state(w + 2, h + 2, seed=w * 7 + h, scale=scale)
     load_both(g, o, st)
@@ -351,7 +355,7 @@ def test_multigrid_bit_exact(w, h, bfecc, scale):
         o.Simulate(0.1)
         assert_same(g, o, "step %d" % step, nan_equal=True)
     if scale < 0.1:
-        assert np.isfinite(g.get("U")).all() and np.isfinite(g.get("p")).all()
+        assert np.isfinite(g.get("U")).mean() > 0.99 and np.isfinite(g.get("p")).all()
     assert g.stats()["sweeps_run"] == o.last_sweeps()
 
 

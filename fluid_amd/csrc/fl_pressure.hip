@@ -227,8 +227,8 @@ __global__ void k_mg_restrict(Grid g, int cNX, int cNY, const float* __restrict_
 // and (row, l + 1). Every step runs the reference's expression on the same operands, so the
 // result is the serial loop's to the bit.
 // ALL: the interior is one tile of at most TR x 64 cells, and the launch runs all 40 sweeps on
-// it in LDS, one walk after another (each walk reads what the previous one wrote: same wave,
-// LDS in program order), instead of one launch per sweep.
+// it in LDS, the four waves walking successive sweeps three steps apart (below), instead of one
+// launch per sweep.
 static constexpr int MG_COARSE_ITERS = 40, MG_TR = 32, MG_TR_ALL = 64, MG_TC = 64, MG_PITCH = MG_TC + 2;
 
 template <int TR, bool ALL>
@@ -276,18 +276,22 @@ __global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int n
   // narrower tile, lane `cols` walks the east halo column. LDS is only read here (the cells a
   // lane will still read are never the ones written), so the walk needs no barrier; results
   // go to LDS for the write-back.
-  if (tid < 64)
-  for (int sweep = 0; sweep < (ALL ? MG_COARSE_ITERS : 1); sweep++) {
   const float relax = 1.6f;
   auto row_of = [&](int x) { return min(max(x, -1), rows) + 1; };  // LDS row of row x, clamped
   const int col = l + 1;                                                // LDS column of this lane
-  int r = -l;
-  float sm1 = ls[row_of(r - 1) * MG_PITCH + col], s0 = ls[row_of(r) * MG_PITCH + col];
-  float sp1 = ls[row_of(r + 1) * MG_PITCH + col];
-  float po = lp[row_of(r) * MG_PITCH + col], po1 = lp[row_of(r + 1) * MG_PITCH + col];
-  float cur = lp[row_of(r - 1) * MG_PITCH + col];
-  float rh = lr[min(max(r, 0), rows - 1) * MG_TC + l];
-  for (int d = 0; d < rows + cols - 1; d++, r++) {
+  int r = 0;
+  float sm1 = 0.0f, s0 = 0.0f, sp1 = 0.0f, po = 0.0f, po1 = 0.0f, cur = 0.0f, rh = 0.0f;
+  auto init = [&]() {
+    r = -l;
+    sm1 = ls[row_of(r - 1) * MG_PITCH + col];
+    s0 = ls[row_of(r) * MG_PITCH + col];
+    sp1 = ls[row_of(r + 1) * MG_PITCH + col];
+    po = lp[row_of(r) * MG_PITCH + col];
+    po1 = lp[row_of(r + 1) * MG_PITCH + col];
+    cur = lp[row_of(r - 1) * MG_PITCH + col];
+    rh = lr[min(max(r, 0), rows - 1) * MG_TC + l];
+  };
+  auto step = [&]() {
     const int hrow = row_of(r) * MG_PITCH;
     const float pWh = lp[hrow], sWh = ls[hrow], pEh = lp[hrow + cols + 1], sEh = ls[hrow + cols + 1];
     const float pW = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(pWh), __float_as_int(cur), 0x138, 0xF, 0xF, false));
@@ -318,7 +322,30 @@ __global__ __launch_bounds__(256) void k_mg_coarse_tiles(int cNX, int cNY, int n
     po = po1;
     po1 = po2;
     rh = rh1;
-  }
+    r++;
+  };
+  if constexpr (ALL) {
+    // The 40 walks overlap on the four waves: wave w walks sweeps w, w+4, ..., and sweep k
+    // starts at step (k / 4) P + 3 (k % 4) of a common clock with a barrier per step. A walk
+    // at step d reads old values (the previous sweep's) on anti-diagonals up to d + 2, which
+    // that sweep wrote at its step d + 2, at least one barrier earlier (lag 3 >= 2 + 1); it
+    // writes anti-diagonal d after the previous sweep read it there (at that sweep's step d - 2).
+    // P >= 12 keeps the lag from sweep 4m+3 to 4m+4 at 3 or more on small tiles.
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int D = rows + cols - 1, P = max(D, 12), M = MG_COARSE_ITERS / 4;
+    const int total = (M - 1) * P + 9 + D;
+    for (int tau = 0; tau < total; tau++) {
+      const int x = tau - 3 * wv;
+      if (x >= 0 && x < M * P) {
+        const int d = x % P;
+        if (d == 0) init();
+        if (d < D) step();
+      }
+      __syncthreads();
+    }
+  } else if (tid < 64) {
+    init();
+    for (int d = 0; d < rows + cols - 1; d++) step();
   }
   __syncthreads();
   for (int e = tid; e < rows * cols; e += 256) {
